@@ -1,0 +1,9 @@
+"""MI355X-native CG-SENSE engine for respiratory-resolved 3D wave-CAIPI MRI.
+
+The hot path of expRad/4d_lung's CGSense program -- the encoding operator E,
+its adjoint E^H and the CG-SENSE loop -- as hand-written HIP kernels for
+gfx950 behind the C-ABI of include/cgs.h.  ``cgs`` is the Python binding of
+that ABI, ``synth`` the seeded synthetic workload generator.
+"""
+from . import cgs, synth  # noqa: F401
+from .cgs import CgsError, Context, Fourier, SOLVE_BATCHED, SOLVE_PLAIN  # noqa: F401

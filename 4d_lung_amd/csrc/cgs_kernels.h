@@ -1,0 +1,141 @@
+// cgs_kernels.h -- kernel argument blocks and host-side launchers of
+// libcgs_hip.so.  Everything here is internal: the public surface is the
+// C-ABI in include/cgs.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "cgs_internal.h"
+
+namespace cgs {
+
+// ------------------------------------------------------------------ FFT
+// Strided line pass: element (outer o, line index l, column c) lives at
+// g[o*os + l*ls + c], c < ncols.
+struct PlainArgs {
+    float2* g;
+    long long os, ls, ncols;
+    const float2* tw;      // exp(-2 pi i n / L), n < L
+    long long outer_per_bin;
+    const int* done;       // per-bin "solve finished" flags of the launch's bins, or null
+};
+
+// z pass (lines along z, columns = coils) with fused prologue/epilogue.
+struct ZArgs {
+    float2* g;             // [bin][Ng][C]
+    const float2* tw;
+    const float2* p;       // [bin][Ng] image vector (zpre expand; zpost <p,q>)
+    const float2* maps;    // [Ng][C]
+    const float* kft;      // [Ng] deapodization map (ConvKernel::kernelFT)
+    float2* q;             // [bin][Ng] (zpost combine)
+    double* part;          // [nblocks][2] partial <p, q> (zpost combine), or null
+    float scale;           // zpost: 1e-3 / sqrt(Ng)
+    int C;
+    long long ng;
+    long long nxy;         // Nx*Ny
+    int ny;
+    int crop;              // reduceFOV active
+    int lo[3], hi[3];      // kept box, inclusive on both ends
+    const int* done;       // per-bin flags of the launch's bins, or null
+};
+
+bool fft_len_supported(int L);
+int zpass_w(int L, int C);
+hipError_t launch_fft_plain(int L, int dir, const PlainArgs& a, long long nouter, hipStream_t s);
+hipError_t launch_fft_z1(int L, int dir, float2* g, long long nlines, const float2* tw, hipStream_t s);
+hipError_t launch_fft_zpre(int L, int W, const ZArgs& a, long long nouter, hipStream_t s);
+hipError_t launch_fft_zpost(int L, int W, const ZArgs& a, long long nouter, hipStream_t s);
+
+// ------------------------------------------------------ KB setup / binning
+// Per-axis constants of Reco::ConvKernel::convFunction (Reco.h:221-226), all
+// computed in float exactly as the reference does.
+struct AxisConst {
+    float dkf;     // float(1.0 / float(N-1))
+    float cut;     // float(w/2) * dkf
+    float wdk;     // w * dkf
+    float denom;   // (100 * w) * dkf
+};
+
+struct KbParams {
+    AxisConst ax[3];
+    double b;      // double(float b)
+    double thr;    // double(float(convFunction(0))) * 1e-7 (Reco.cpp:716-717)
+    int h;         // window half width (taps at nearest-h .. nearest+h)
+    int kw;        // 2h+1
+    int order;     // reference search order (Reco.cpp:380)
+};
+
+hipError_t launch_bin_keys(const double* traj, long long ns, const float2* data0, const uint8_t* mask,
+                           Geom g, Tiles t, int* key, int* nearest, int* counts, hipStream_t s);
+hipError_t launch_scan_offsets(const int* counts, int n, int* off, hipStream_t s);
+hipError_t launch_scatter(const int* key, long long ns, const int* off, int* cursor, int* tmp, hipStream_t s);
+hipError_t launch_segment_rank(const int* tmp, const int* off, int ntiles, int* perm, hipStream_t s);
+hipError_t launch_sample_tables(const double* traj, const int* perm, long long m, Geom g, KbParams kb,
+                                int4* base, double* vals, hipStream_t s);
+hipError_t launch_kernel_ft(const double* axr, const double* axi, Geom g, float* kft, hipStream_t s);
+hipError_t launch_gather_data(const float2* data, long long ns, const int4* base, long long m, int C,
+                              const float* dcf_full, const int* perm, float* dcf_sorted, float2* y, hipStream_t s);
+hipError_t launch_scatter_data(const float2* y, const int4* base, long long m, int C, long long ns,
+                               float2* data, hipStream_t s);
+hipError_t launch_transpose_coils(const float2* in, float2* out, long long ng, int C, int to_interleaved,
+                                  hipStream_t s);
+
+// ---------------------------------------------------------- gridding
+// Samples of all bins are concatenated; bin b owns [bin_off[b], bin_off[b+1]).
+// A launch covers the bins b0 .. b0+nb-1, whose samples are contiguous.
+struct SampleArgs {
+    const int4* base;      // [Mtot] (bx, by, bz, original sample index)
+    const double* vals;    // [Mtot][val_stride(kw)]
+    const float* dcf;      // [Mtot] or null (density weighting applied by degrid)
+    const int* sbin;       // [Mtot] global bin of each sample
+    long long first;       // first sample of the launch
+    long long m;           // samples in the launch
+};
+
+struct GridArgs {
+    float2* G;             // [nb][Ng][C] coil grids of the launch's bins
+    float2* Y;             // [Mtot][C] sample values (global indexing)
+    SampleArgs s;
+    Geom g;
+    Tiles t;
+    const int* tile_off;   // [nbins_total][ntiles+1] absolute sample offsets
+    const float* gdcf;     // gridding only: multiply Y by these weights first (or null)
+    double thr;
+    int b0, nb;
+    const int* done;       // [nbins_total] flags, or null
+};
+
+hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s);
+hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s);
+int grid_tile_configs();
+void grid_tile_dims(int tcfg, int* tx, int* ty, int* tz);
+
+// ---------------------------------------------------------------- CG
+struct CgScalars {
+    double rr[2];      // <r, r>
+    double pq[2];      // <p, q>
+    double alpha[2];
+    double beta[2];
+    double resid;      // ||r||_2 after the last update
+    int iter;          // completed iterations
+    int done;          // no further iterations for this bin
+    int nonfinite;     // q[0] was NaN/Inf (Reco.cpp:264-268)
+    int pad;
+};
+
+hipError_t launch_dot_partials(const float2* a, const float2* b, long long n, int nbins, int nblk, double* part,
+                               hipStream_t s);
+hipError_t launch_cg_init(CgScalars* sc, const double* part, int nblk, int nbins, double thresh, int* done,
+                          hipStream_t s);
+hipError_t launch_cg_alpha(CgScalars* sc, const double* part, int nblk, int nbins, const float2* q,
+                           long long ng, int* done, hipStream_t s);
+hipError_t launch_cg_update_xr(float2* x, float2* r, const float2* p, const float2* q, long long n, int nbins,
+                               int nblk, const CgScalars* sc, double* part, hipStream_t s);
+hipError_t launch_cg_beta(CgScalars* sc, const double* part, int nblk, int nbins, double thresh, int max_iter,
+                          double* resid_hist, int hist_stride, int* done, hipStream_t s);
+hipError_t launch_cg_update_p(float2* p, const float2* r, long long n, int nbins, const CgScalars* sc,
+                              hipStream_t s);
+hipError_t launch_copy_rhs(float2* x, float2* r, float2* p, const float2* b, long long n, int nbins, hipStream_t s);
+hipError_t launch_fill_zero(float2* a, long long n, hipStream_t s);
+hipError_t launch_coil_combine(const float2* G, const float2* maps, float2* q, long long ng, int C, hipStream_t s);
+hipError_t launch_scale_d(float2* a, long long n, double f, hipStream_t s);
+
+}  // namespace cgs

@@ -1,0 +1,268 @@
+// cgs_nufft.hip -- Kaiser-Bessel degridding (gather) and gridding (tile
+// accumulation) over the coil-interleaved grid.
+//
+// The tap set and weights are the reference's (Reco.cpp:698-787): for a sample
+// with per-axis factors vx, vy, vz (FP64), the weight of grid cell
+// (bx+ox, by+oy, bz+oz) is w = float((vx[ox]*vy[oy])*vz[oz]) and the tap is kept
+// when double(w) > thr; taps are visited x-major, then y, then z, like the
+// candidate list of Reco.cpp:719-729.
+//
+// Degridding (nufftFWD, Reco.cpp:539-550; DCF of Reco.cpp:243-254 fused):
+// one lane per (sample, coil); the lanes of a sample read the C consecutive
+// coil values of each tap cell, so every tap is one coalesced C*8-byte load.
+// Samples are tile-sorted, so neighbouring samples hit the same L2 lines.
+//
+// Gridding (nufftBWD, Reco.cpp:484-504): one workgroup owns one TX*TY*TZ
+// tile of cells for all coils and keeps it in registers (no atomics, no
+// halo write-back).  It scans the samples of the 27 neighbouring tiles (9
+// contiguous ranges of the tile-sorted array), compacts those whose KB window
+// meets the tile into an LDS list in a fixed order, and every thread adds the
+// contributions to the (column, coil) pairs it owns.  The result is
+// deterministic and every cell is written exactly once.
+#include "cgs_internal.h"
+#include "cgs_kernels.h"
+
+namespace cgs {
+
+static inline int pow2_at_least(int c, int cap) {
+    int p = 1;
+    while (p < c && p < cap) p <<= 1;
+    return p;
+}
+
+// ------------------------------------------------------------ degridding
+template <int KW, int CW>
+__global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
+    constexpr int VS = val_stride(KW);
+    const long long e = (long long)blockIdx.x * CGS_THREADS + threadIdx.x;
+    const long long s = a.s.first + e / CW;
+    const int lane_c = (int)(e % CW);
+    if (e / CW >= a.s.m) return;
+    const int bin = a.s.sbin[s];
+    if (a.done && a.done[bin]) return;
+    const int4 bs = a.s.base[s];
+    const double* __restrict__ v = a.s.vals + s * VS;
+    const int C = a.g.C;
+    const float2* __restrict__ G = a.G + (long long)(bin - a.b0) * a.g.ng * C;
+    const float d = a.s.dcf ? a.s.dcf[s] : 1.0f;
+    for (int c = lane_c; c < C; c += CW) {
+        float2 acc = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int ox = 0; ox < KW; ++ox) {
+            const double vx = v[ox];
+            if (vx == 0.0) continue;
+            const long long xo = (long long)(bs.x + ox) * a.g.ny;
+#pragma unroll
+            for (int oy = 0; oy < KW; ++oy) {
+                const double vy = v[KW + oy];
+                if (vy == 0.0) continue;
+                const double vxy = vx * vy;
+                const long long row = (xo + bs.y + oy) * a.g.nz + bs.z;
+#pragma unroll
+                for (int oz = 0; oz < KW; ++oz) {
+                    const float w = (float)(vxy * v[2 * KW + oz]);
+                    if ((double)w > a.thr) {
+                        const float2 gv = G[(row + oz) * C + c];
+                        acc.x = fmaf(w, gv.x, acc.x);
+                        acc.y = fmaf(w, gv.y, acc.y);
+                    }
+                }
+            }
+        }
+        a.Y[s * C + c] = make_float2(acc.x * d, acc.y * d);
+    }
+}
+
+// -------------------------------------------------------------- gridding
+template <int KW, int TX, int TY, int TZ, int CP>
+__global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
+    constexpr int VS = val_stride(KW);
+    constexpr int NCOL = TX * TY;
+    constexpr int NPAIR = NCOL * CP;
+    constexpr int NPT = (NPAIR + CGS_THREADS - 1) / CGS_THREADS;
+    constexpr int NWAVE = CGS_THREADS / 64;
+    __shared__ int list[CGS_THREADS];
+    __shared__ int wcount[NWAVE];
+
+    const int ntiles = a.t.ntiles;
+    const int lbin = blockIdx.x / ntiles;
+    const int tile = blockIdx.x % ntiles;
+    const int bin = a.b0 + lbin;
+    if (a.done && a.done[bin]) return;
+    const int itz = tile % a.t.ntz;
+    const int ity = (tile / a.t.ntz) % a.t.nty;
+    const int itx = tile / (a.t.ntz * a.t.nty);
+    const int x0 = itx * TX, y0 = ity * TY, z0 = itz * TZ;
+    const int C = a.g.C;
+    const int* __restrict__ off = a.tile_off + (long long)bin * (ntiles + 1);
+
+    float2 acc[NPT][TZ];
+#pragma unroll
+    for (int k = 0; k < NPT; ++k)
+#pragma unroll
+        for (int z = 0; z < TZ; ++z) acc[k][z] = make_float2(0.0f, 0.0f);
+
+    const int tid = threadIdx.x;
+    const int lane = tid % 64, wid = tid / 64;
+    const int tz_lo = itz > 0 ? itz - 1 : 0;
+    const int tz_hi = itz + 1 < a.t.ntz ? itz + 1 : a.t.ntz - 1;
+
+    for (int dx = -1; dx <= 1; ++dx) {
+        const int jx = itx + dx;
+        if (jx < 0 || jx >= a.t.ntx) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int jy = ity + dy;
+            if (jy < 0 || jy >= a.t.nty) continue;
+            const int tcol = (jx * a.t.nty + jy) * a.t.ntz;
+            const int rbeg = off[tcol + tz_lo];
+            const int rend = off[tcol + tz_hi + 1];
+            for (int start = rbeg; start < rend; start += CGS_THREADS) {
+                // --- compact the candidates whose window meets this tile
+                const int s = start + tid;
+                bool hit = false;
+                if (s < rend) {
+                    const int4 bs = a.s.base[s];
+                    hit = bs.x < x0 + TX && bs.x + KW > x0 && bs.y < y0 + TY && bs.y + KW > y0 &&
+                          bs.z < z0 + TZ && bs.z + KW > z0;
+                }
+                const unsigned long long bal = __ballot(hit);
+                const int wpre = __popcll(bal & ((1ULL << lane) - 1ULL));
+                if (lane == 0) wcount[wid] = __popcll(bal);
+                __syncthreads();
+                int wbase = 0, total = 0;
+#pragma unroll
+                for (int w = 0; w < NWAVE; ++w) {
+                    if (w < wid) wbase += wcount[w];
+                    total += wcount[w];
+                }
+                if (hit) list[wbase + wpre] = s;
+                __syncthreads();
+                // --- accumulate
+                for (int j = 0; j < total; ++j) {
+                    const int sj = list[j];
+                    const int4 bs = a.s.base[sj];
+                    const double* __restrict__ v = a.s.vals + (long long)sj * VS;
+#pragma unroll
+                    for (int k = 0; k < NPT; ++k) {
+                        const int e = tid + k * CGS_THREADS;
+                        const int c = e % CP, col = e / CP;
+                        if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
+                            const int ox = x0 + col / TY - bs.x;
+                            const int oy = y0 + col % TY - bs.y;
+                            if (ox >= 0 && ox < KW && oy >= 0 && oy < KW) {
+                                const double vxy = v[ox] * v[KW + oy];
+                                if (vxy != 0.0) {
+                                    float2 yv = a.Y[(long long)sj * C + c];
+                                    if (a.gdcf) yv = cscale(yv, a.gdcf[sj]);
+#pragma unroll
+                                    for (int z = 0; z < TZ; ++z) {
+                                        const int oz = z0 + z - bs.z;
+                                        if (oz >= 0 && oz < KW) {
+                                            const float w = (float)(vxy * v[2 * KW + oz]);
+                                            if ((double)w > a.thr) {
+                                                acc[k][z].x = fmaf(w, yv.x, acc[k][z].x);
+                                                acc[k][z].y = fmaf(w, yv.y, acc[k][z].y);
+                                            }
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // --- write the tile (every cell of the grid belongs to exactly one tile)
+    float2* __restrict__ G = a.G + (long long)lbin * a.g.ng * C;
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int e = tid + k * CGS_THREADS;
+        const int c = e % CP, col = e / CP;
+        if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
+            const int x = x0 + col / TY, y = y0 + col % TY;
+            if (x < a.g.nx && y < a.g.ny) {
+                const long long row = ((long long)x * a.g.ny + y) * a.g.nz;
+#pragma unroll
+                for (int z = 0; z < TZ; ++z)
+                    if (z0 + z < a.g.nz) G[(row + z0 + z) * C + c] = acc[k][z];
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------------- dispatch
+hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s) {
+    const int C = a.g.C;
+    const int cw = pow2_at_least(C, 64);
+    const long long nthr = a.s.m * cw;
+    if (a.s.m == 0) return hipSuccess;
+    const dim3 grid((unsigned)((nthr + CGS_THREADS - 1) / CGS_THREADS));
+#define CGS_DEG(KWv, CWv) k_degrid<KWv, CWv><<<grid, CGS_THREADS, 0, s>>>(a)
+#define CGS_DEG_CW(KWv)                        \
+    switch (cw) {                              \
+        case 1: CGS_DEG(KWv, 1); break;        \
+        case 2: CGS_DEG(KWv, 2); break;        \
+        case 4: CGS_DEG(KWv, 4); break;        \
+        case 8: CGS_DEG(KWv, 8); break;        \
+        case 16: CGS_DEG(KWv, 16); break;      \
+        case 32: CGS_DEG(KWv, 32); break;      \
+        default: CGS_DEG(KWv, 64); break;      \
+    }
+    switch (kw) {
+        case 3: CGS_DEG_CW(3); break;
+        case 5: CGS_DEG_CW(5); break;
+        case 7: CGS_DEG_CW(7); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef CGS_DEG_CW
+#undef CGS_DEG
+    return hipGetLastError();
+}
+
+// Gridding tile shapes (cells per tile along x, y, z).
+static const int kTileCfg[][3] = {{8, 8, 8}, {8, 8, 4}, {4, 8, 8}, {4, 4, 8}};
+
+int grid_tile_configs() { return (int)(sizeof(kTileCfg) / sizeof(kTileCfg[0])); }
+
+void grid_tile_dims(int tcfg, int* tx, int* ty, int* tz) {
+    *tx = kTileCfg[tcfg][0];
+    *ty = kTileCfg[tcfg][1];
+    *tz = kTileCfg[tcfg][2];
+}
+
+template <int KW, int TX, int TY, int TZ>
+static hipError_t grid_cp(const GridArgs& a, hipStream_t s) {
+    const int cp = pow2_at_least(a.g.C, 64);
+    const dim3 grid((unsigned)((long long)a.nb * a.t.ntiles));
+    switch (cp) {
+        case 1: case 2: case 4: case 8: k_grid<KW, TX, TY, TZ, 8><<<grid, CGS_THREADS, 0, s>>>(a); break;
+        case 16: k_grid<KW, TX, TY, TZ, 16><<<grid, CGS_THREADS, 0, s>>>(a); break;
+        case 32: k_grid<KW, TX, TY, TZ, 32><<<grid, CGS_THREADS, 0, s>>>(a); break;
+        default: return hipErrorInvalidValue;   // more than 32 coils: not supported by gridding
+    }
+    return hipGetLastError();
+}
+
+template <int KW>
+static hipError_t grid_tile(int tcfg, const GridArgs& a, hipStream_t s) {
+    switch (tcfg) {
+        case 0: return grid_cp<KW, 8, 8, 8>(a, s);
+        case 1: return grid_cp<KW, 8, 8, 4>(a, s);
+        case 2: return grid_cp<KW, 4, 8, 8>(a, s);
+        case 3: return grid_cp<KW, 4, 4, 8>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s) {
+    switch (kw) {
+        case 3: return grid_tile<3>(tcfg, a, s);
+        case 5: return grid_tile<5>(tcfg, a, s);
+        case 7: return grid_tile<7>(tcfg, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cgs

@@ -1,0 +1,150 @@
+/* cgs.h -- C-ABI of libcgs_hip.so, the MI355X (gfx950) CG-SENSE engine.
+ *
+ * Drop-in boundary for the hot path of expRad/4d_lung's CGSense program: the
+ * encoding operator E (Reco::nufftFWD), its adjoint E^H (Reco::nufftBWD), the
+ * centred 3D FFT (Fourier::transformFWD/BWD) and the CG-SENSE loop
+ * (Reco::cgSenseDynamic).  Plain pointers and sizes only; every call returns a
+ * status (0 = OK) and never exits the process -- where the reference prints and
+ * calls exit(EXIT_FAILURE) (Reco.cpp:194-195, 266-267, 333-334, 475-476) this
+ * library returns a non-zero status and keeps a message for cgs_last_error().
+ *
+ * Layouts are the reference's (host side):
+ *   complex values   float[2] interleaved (fftwf_complex)
+ *   images / maps    [x][y][z], z fastest; several coils: [c][x][y][z] (IO.cpp:25-51)
+ *   k-space data     [c][i] with i = ix*(sY*sZ) + iy*sZ + iz, readout ix slowest,
+ *                    zero-filled where a line is not acquired (IO.cpp:55-98)
+ *   trajectory       [i][3] doubles in [0,1] per axis (Trajectory.h:32)
+ *   density weights  [i] floats in the same sample order (IO.cpp:129-156)
+ *
+ * A context is used from one host thread at a time.  Host buffers are only
+ * borrowed for the duration of a call.
+ */
+#ifndef CGS_H
+#define CGS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cgs_ctx cgs_ctx;
+typedef struct cgs_fft cgs_fft;
+
+#define CGS_OK 0
+#define CGS_ERR_ARG 1         /* invalid argument or unsupported size */
+#define CGS_ERR_HIP 2         /* HIP runtime error */
+#define CGS_ERR_STATE 3       /* call out of order (e.g. no trajectory yet) */
+#define CGS_ERR_NONFINITE 4   /* NaN/Inf in q[0] during CG (Reco.cpp:264-268) */
+#define CGS_ERR_MEMORY 5      /* device allocation failed */
+
+#define CGS_SOLVE_PLAIN 0     /* one bin at a time, host reads the residual every iteration */
+#define CGS_SOLVE_BATCHED 1   /* all bins per launch, device-side loop control, hipGraph replay */
+
+/* Message of the last failing call on this context (never NULL). */
+const char* cgs_last_error(const cgs_ctx* h);
+
+/* Reco::setGridSize / setSampling / setNCh + Reco::init (Reco.cpp:374-400,
+ * 415-427): grid = oversampled grid (x,y,z), samp = (sX, sY, sZ).  KB constants
+ * default to the reference's w = 4.25, b = 14.9086, order 5 (Reco.cpp:378-380),
+ * imgRange to the grid (Reco.cpp:397). */
+int cgs_create(const int grid[3], const int samp[3], int ncoil, cgs_ctx** out);
+/* Reco::~Reco (Reco.cpp:29-61). */
+int cgs_destroy(cgs_ctx* h);
+
+/* ConvKernel::setW / setB / setKernelOrder (Reco.h:54-56). */
+int cgs_set_kernel(cgs_ctx* h, float w, float b, int order);
+/* reduceFOV box (Reco.cpp:626-671; imgRange Reco.h:195, offcenter Reco.cpp:429-434). */
+int cgs_set_img_range(cgs_ctx* h, const int range[3], const int offcenter[3]);
+/* Gridding tile shape index (0..cgs_num_tile_configs()-1), chosen by the autotuner. */
+int cgs_set_tile_config(cgs_ctx* h, int tcfg);
+int cgs_num_tile_configs(void);
+int cgs_get_tile_config(const cgs_ctx* h, int* tcfg, int dims[3]);
+
+/* Trajectory::kSpaceTraj, as Reco::setTrajectory leaves it (Reco.cpp:324-336). */
+int cgs_set_trajectory(cgs_ctx* h, const double* ktraj);
+/* ConvKernel::calculateWeights (Reco.cpp:698-787): every sample active. */
+int cgs_calculate_weights(cgs_ctx* h);
+/* ConvKernel::excludeMissingData (Reco.cpp:681-696): samples whose coil-0
+ * value in data ([c][i] complex) is exactly 0+0i are dropped. */
+int cgs_exclude_missing_data(cgs_ctx* h, const float* data);
+/* ConvKernel::calculateKernelFT (Reco.cpp:560-601). */
+int cgs_calculate_kernel_ft(cgs_ctx* h);
+int cgs_get_kernel_ft(cgs_ctx* h, float* kft /* Ng floats (real part) */);
+/* Reco::denCorr (Reco.h:86); NULL = Density::NONE. */
+int cgs_set_density(cgs_ctx* h, const float* dcf);
+/* Reco::sMaps (IO::importSensitivityMaps, IO.cpp:25-51): [c][x][y][z] complex. */
+int cgs_set_maps(cgs_ctx* h, const float* maps);
+
+/* Reco::nufftFWD (Reco.cpp:521-553): img [c][Ng] complex -> data [c][i]
+ * complex (samples that are not active are written as 0). */
+int cgs_nufft_fwd(cgs_ctx* h, const float* img, float* data);
+/* Reco::nufftBWD (Reco.cpp:468-519): data [c][i] -> img [c][Ng]. */
+int cgs_nufft_bwd(cgs_ctx* h, const float* data, float* img);
+/* One CG system matrix product q = sum_c conj(S_c) E^H W E (S_c p)
+ * (Reco.cpp:232-262); p, q: Ng complex. */
+int cgs_apply_normal(cgs_ctx* h, const float* p, float* q);
+
+/* Reco::cgSenseDynamic (Reco.cpp:175-321) started from coil images img
+ * [c][Ng] (the reference's Reco::img after nufftBWD).  At most max_iter + 1
+ * passes (the reference's nMaxIter = 15, Reco.cpp:228).  x_out: Ng complex;
+ * resid_out: max_iter + 1 doubles (Reco::cgResiduals); iters_out: passes made
+ * (Reco::numPerformedCGIterations). */
+int cgs_cg_sense_images(cgs_ctx* h, const float* img, double thresh, int max_iter, float* x_out,
+                        double* resid_out, int* iters_out);
+
+/* ---- respiratory bins (all bins share trajectory, maps and KB tables) ---- */
+/* Number of datasets (respiratory bins) held by the context (default 1). */
+int cgs_set_num_bins(cgs_ctx* h, int nbins);
+/* Bin data in the reference's zero-filled layout (one state{n}.bin,
+ * CGSense.cpp:200-226): active samples = coil-0 value != 0; dcf may be NULL
+ * (then the context's density weights, if any, are used). */
+int cgs_set_bin_data(cgs_ctx* h, int bin, const float* data, const float* dcf);
+/* Compact form: m acquired samples with original indices idx[m] (any order),
+ * values y[c][m] complex and optional weights dcf[m]. */
+int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, const float* y, const float* dcf);
+/* Right-hand sides b = E^H W y of every bin and CG state reset. */
+int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter);
+/* Run CG iterations on all bins (mode CGS_SOLVE_PLAIN or CGS_SOLVE_BATCHED)
+ * until every bin stopped or n_iter iterations were issued; returns after the
+ * device finished. */
+int cgs_cg_iterate(cgs_ctx* h, int mode, int n_iter);
+/* Results: x_out nbins*Ng complex (may be NULL), resid_out nbins*(max_iter+1),
+ * iters_out nbins. */
+int cgs_cg_results(cgs_ctx* h, float* x_out, double* resid_out, int* iters_out);
+/* prepare + iterate(max_iter + 1) + results. */
+int cgs_cg_sense_bins(cgs_ctx* h, int mode, double thresh, int max_iter, float* x_out, double* resid_out,
+                      int* iters_out);
+
+/* ---- setup introspection (bit-exactness tests) ---- */
+/* Active samples of a bin and their tile-sorted order. */
+int cgs_bin_count(cgs_ctx* h, int bin, long long* m);
+/* nearest[i][3] of every sample, perm[m] original index per sorted position,
+ * tile_off[ntiles+1], base[m][3] window origins, vals[m][3*kw] per-axis KB
+ * factors (FP64); any pointer may be NULL. */
+int cgs_bin_tables(cgs_ctx* h, int bin, int32_t* nearest, int32_t* perm, int32_t* tile_off, int32_t* base,
+                   double* vals);
+int cgs_kernel_info(const cgs_ctx* h, int* kw, int* ntiles, double* thr);
+
+/* ---- Fourier (Fourier.h:12-54) on one complex grid ---- */
+int cgs_fft_create(const int dims[3], cgs_fft** out);
+int cgs_fft_destroy(cgs_fft* f);
+/* Fourier::transformFWD (Fourier.cpp:19-24), in place on a host array. */
+int cgs_fft_forward(cgs_fft* f, float* a);
+/* Fourier::transformBWD (Fourier.cpp:26-42), in place on a host array. */
+int cgs_fft_backward(cgs_fft* f, float* a, int rescale);
+
+/* ---- timing ---- */
+/* Per-kernel HIP-event timing on the context's stream (off by default). */
+int cgs_profile_enable(cgs_ctx* h, int on);
+/* Writes "name count total_ms\n" lines; returns the needed length. */
+int cgs_profile_report(cgs_ctx* h, char* buf, int len);
+int cgs_profile_reset(cgs_ctx* h);
+/* Device bytes held by the context. */
+long long cgs_device_bytes(const cgs_ctx* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGS_H */

@@ -1,0 +1,172 @@
+"""Parity of the HIP path (libcgs_hip.so through its C-ABI) with the CPU
+oracle on the same seeded inputs.
+
+Bars: integer binning and KB tables bit-exact; FP32 operators within the
+stated relative L2 of the FP64 oracle (the FP32 reference arithmetic of
+Reco.cpp itself is a few 1e-7 away from FP64 per operator application); the
+CG image within 1e-4 relative L2 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from problems import config_problem, gpu_context, small_problem
+
+pytestmark = pytest.mark.gpu
+
+FFT_TOL = 2e-6       # FP32 Stockham vs FP64 DFT, relative L2
+OP_TOL = 2e-6        # one E / E^H application, relative L2
+CG_TOL = 1e-4        # CG-SENSE image after the configured iterations
+
+
+@pytest.fixture(scope="module")
+def small(orc):
+    return small_problem(orc)
+
+
+@pytest.fixture(scope="module")
+def small_dcf(orc):
+    return small_problem(orc, seed=11, dcf=True)
+
+
+@pytest.mark.parametrize("shape", [(8, 12, 16), (16, 24, 20), (32, 28, 40), (64, 48, 56), (128, 128, 96),
+                                   (96, 112, 160)])
+def test_fft_matches_oracle(pkg, orc, shape):
+    rng = np.random.default_rng(sum(shape))
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    f = pkg.Fourier(shape)
+    ref_f = orc.fft3_centered(a.astype(np.complex128), -1)
+    ref_b = orc.fft3_centered(a.astype(np.complex128), +1, rescale=True)
+    assert rel_l2(f.forward(a), ref_f) < FFT_TOL
+    assert rel_l2(f.backward(a, rescale=True), ref_b) < FFT_TOL
+    ref_b0 = orc.fft3_centered(a.astype(np.complex128), +1, rescale=False)
+    assert rel_l2(f.backward(a, rescale=False), ref_b0) < FFT_TOL
+
+
+def test_kernel_ft_bit_exact(pkg, orc, small):
+    ctx = gpu_context(pkg, small)
+    np.testing.assert_array_equal(ctx.kernel_ft(), small["oracle"].get_kernel_ft())
+
+
+@pytest.mark.parametrize("tcfg", [0, 1, 2, 3])
+def test_binning_and_tables_bit_exact(pkg, orc, small, tcfg):
+    P = small
+    ctx = gpu_context(pkg, P, tcfg=tcfg)
+    grid = np.array(P["grid"])
+    T = ctx.bin_tables(0)
+    kw, ntiles, thr = ctx.kernel_info()
+    assert thr == P["oracle"].kernel_threshold()
+    # nearest grid point (int)((N-1)k + 0.5) in double, for every sample
+    near = np.floor((grid[None, :] - 1) * P["traj"] + 0.5).astype(np.int64)
+    np.testing.assert_array_equal(T["nearest"], near)
+    # active set = acquired samples, in stable tile order
+    active = np.flatnonzero(P["data"][0] != 0)
+    _, dims = ctx.tile_config()
+    nc = np.clip(near[active], 0, grid - 1) // np.array(dims)
+    ntile = -(-grid // np.array(dims))
+    key = (nc[:, 0] * ntile[1] + nc[:, 1]) * ntile[2] + nc[:, 2]
+    order = np.argsort(key, kind="stable")
+    np.testing.assert_array_equal(T["perm"], active[order])
+    np.testing.assert_array_equal(T["tile_off"], np.concatenate([[0], np.cumsum(np.bincount(key, minlength=ntiles))]))
+    # expanding the separable tables reproduces the reference tap lists
+    counts, idx, w = P["oracle"].get_weights()
+    starts = np.concatenate([[0], np.cumsum(counts)])
+    N = P["grid"]
+    for s, i in enumerate(T["perm"][:400]):
+        bx, by, bz = T["base"][s]
+        v = T["vals"][s]
+        taps, wts = [], []
+        for ox in range(kw):
+            for oy in range(kw):
+                vxy = v[ox] * v[kw + oy]
+                for oz in range(kw):
+                    wf = np.float32(vxy * v[2 * kw + oz])
+                    if float(wf) > thr:
+                        taps.append((bz + oz) + N[2] * (by + oy) + N[1] * N[2] * (bx + ox))
+                        wts.append(wf)
+        np.testing.assert_array_equal(np.array(taps), idx[starts[i]:starts[i + 1]])
+        np.testing.assert_array_equal(np.array(wts, np.float32), w[starts[i]:starts[i + 1]])
+
+
+def test_nufft_bwd_matches_oracle(pkg, orc, small):
+    ctx = gpu_context(pkg, small)
+    got = ctx.nufft_bwd(small["data"])
+    ref = small["oracle"].nufft_bwd(small["data"].astype(np.complex128), prec=64)
+    assert rel_l2(got, ref) < OP_TOL
+
+
+def test_nufft_fwd_matches_oracle(pkg, orc, small):
+    ctx = gpu_context(pkg, small)
+    rng = np.random.default_rng(3)
+    img = (rng.standard_normal((small["ncoil"], int(np.prod(small["grid"])))) +
+           1j * rng.standard_normal((small["ncoil"], int(np.prod(small["grid"]))))).astype(np.complex64)
+    got = ctx.nufft_fwd(img)
+    ref = small["oracle"].nufft_fwd(img.astype(np.complex128), prec=64)
+    assert rel_l2(got, ref) < OP_TOL
+    # samples that are not acquired come back as exact zeros
+    assert np.all(got[:, small["data"][0] == 0] == 0)
+
+
+@pytest.mark.parametrize("which", ["small", "small_dcf"])
+def test_apply_normal_matches_oracle(pkg, orc, which, request):
+    P = request.getfixturevalue(which)
+    ctx = gpu_context(pkg, P)
+    rng = np.random.default_rng(5)
+    p = (rng.standard_normal(int(np.prod(P["grid"]))) + 1j * rng.standard_normal(int(np.prod(P["grid"])))).astype(
+        np.complex64)
+    got = ctx.apply_normal(p)
+    ref = P["oracle"].apply_normal(p.astype(np.complex128), prec=64)
+    assert rel_l2(got, ref) < OP_TOL
+
+
+def test_cg_images_matches_oracle(pkg, orc, small_dcf):
+    """Reco::cgSenseDynamic started from host coil images (reference call order)."""
+    P = small_dcf
+    ctx = gpu_context(pkg, P)
+    o = P["oracle"]
+    dd = P["data"].copy()
+    dd *= P["dcf"][None, :]                  # performDensityCorrection (CGSense.cpp:226)
+    img = ctx.nufft_bwd(dd)                  # nufftBWD (CGSense.cpp:235)
+    x, res, it = ctx.cg_sense_images(img, thresh=0.0, max_iter=9)
+    xr, resr, itr = o.cg_sense(P["data"], thresh=0.0, max_iter=9, prec=64)
+    assert it == itr == 10
+    assert rel_l2(x, xr) < CG_TOL
+    np.testing.assert_allclose(res, resr, rtol=1e-3)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_cg_bins_matches_oracle(pkg, orc, small_dcf, mode):
+    P = small_dcf
+    ctx = gpu_context(pkg, P)
+    ctx.set_bin_data(0, P["data"], P["dcf"])
+    x, res, its = ctx.cg_sense_bins(mode=mode, thresh=0.0, max_iter=9)
+    xr, resr, itr = P["oracle"].cg_sense(P["data"], thresh=0.0, max_iter=9, prec=64)
+    assert its[0] == itr
+    assert rel_l2(x[0], xr) < CG_TOL
+
+
+def test_cg_threshold_stops_like_reference(pkg, orc, small):
+    """Loop control of Reco.cpp:230, 308-309 with an absolute threshold."""
+    P = small
+    ctx = gpu_context(pkg, P)
+    ctx.set_bin_data(0, P["data"])
+    xr, resr, itr = P["oracle"].cg_sense(P["data"], thresh=0.0, max_iter=15, prec=64)
+    thresh = float(np.sqrt(resr[3] * resr[4]))     # stops right after the 5th pass
+    _, res, its = ctx.cg_sense_bins(mode=1, thresh=thresh, max_iter=15)
+    xo, reso, ito = P["oracle"].cg_sense(P["data"], thresh=thresh, max_iter=15, prec=64)
+    assert its[0] == ito == 5
+
+
+def test_plain_64x64x48_cg_matches_oracle(pkg, orc):
+    """The CPU-runnable workload: 64x64x48 image on a 128x128x96 grid, 8 coils,
+    R=2x2 wave-CAIPI, 10 CG iterations."""
+    P = config_problem(orc, "plain_64x64x48")
+    ctx = gpu_context(pkg, P)
+    ctx.set_bin_data(0, P["data"])
+    iters = P["iters"]
+    x, res, its = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=iters - 1)
+    xr, resr, itr = P["oracle"].cg_sense(P["data"], thresh=0.0, max_iter=iters - 1, prec=64)
+    assert its[0] == itr == iters
+    err = rel_l2(x[0], xr)
+    print("plain_64x64x48 rel-L2 vs FP64 oracle: %.3e" % err)
+    assert err < CG_TOL
