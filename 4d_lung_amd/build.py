@@ -16,6 +16,11 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcgs_hip.so")
+RECO_LIB = os.path.join(HERE, "libcgs_reco.so")
+RECO_MAIN = os.path.join(HERE, "cgsense_dropin")
+RECO_SRC = os.path.join(CSRC, "reco")
+RECO_INC = os.path.join(INCLUDE, "cgs_reco")
+CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -52,9 +57,32 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-4000:]))
     os.replace(tmp, LIB)
+    build_reco()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, RECO_LIB, RECO_MAIN)
     return LIB
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
+
+
+def build_reco():
+    """The drop-in Reco / Fourier / IO / Trajectory classes (host C++ over the
+    C-ABI) and the per-state driver program."""
+    flags = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
+             "-I" + INCLUDE, "-I" + RECO_INC]
+    srcs = [os.path.join(RECO_SRC, f) for f in ("Reco.cpp", "Fourier.cpp", "IO.cpp", "Trajectory.cpp")]
+    tmp = RECO_LIB + ".tmp"
+    _run([CXX] + flags + ["-shared", "-o", tmp] + srcs +
+         ["-L" + HERE, "-l:libcgs_hip.so", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, RECO_LIB)
+    tmp = RECO_MAIN + ".tmp"
+    _run([CXX] + flags + ["-o", tmp, os.path.join(RECO_SRC, "cgsense_main.cpp"),
+                          "-L" + HERE, "-l:libcgs_reco.so", "-l:libcgs_hip.so", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, RECO_MAIN)
 
 
 if __name__ == "__main__":

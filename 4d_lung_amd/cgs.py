@@ -181,7 +181,10 @@ class Context:
         self._ck(self._L.cgs_calculate_weights(self.h))
 
     def exclude_missing_data(self, data):
-        d = _c64(data, self.ncoil * self.ns)
+        """Only coil 0 is read: data may be (C, nS) or just (1, nS)."""
+        d = _c64(data)
+        if d.size not in (self.ns, self.ncoil * self.ns):
+            raise ValueError("data must hold nS or C*nS complex values")
         self._ck(self._L.cgs_exclude_missing_data(self.h, _ptr(d)))
 
     def calculate_kernel_ft(self):

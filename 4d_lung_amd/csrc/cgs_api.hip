@@ -93,6 +93,7 @@ struct cgs_ctx {
     int* d_done = nullptr;
     double* d_hist = nullptr;
     int hist_stride = 0;
+    int hist_bins = 0;
     double thresh = 0.0;
     int max_iter = 15;
     bool cg_ready = false;
@@ -100,6 +101,9 @@ struct cgs_ctx {
 
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
+    double g_thresh = 0.0;         // loop-control values the graph was captured with
+    int g_max_iter = -1;
+    double* g_hist = nullptr;
 
     bool prof = false;
     std::vector<ProfEntry> pending;
@@ -972,9 +976,12 @@ int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter) {
     CKS(ensure_grid(h, h->group));
     h->thresh = thresh;
     h->max_iter = max_iter;
-    dfree(h, h->d_hist, 1);
-    h->hist_stride = max_iter + 1;
-    CKS(dalloc(h, &h->d_hist, (size_t)h->nbins * h->hist_stride));
+    if (!h->d_hist || h->hist_stride != max_iter + 1 || h->hist_bins != h->nbins) {
+        dfree(h, h->d_hist, 1);
+        h->hist_stride = max_iter + 1;
+        h->hist_bins = h->nbins;
+        CKS(dalloc(h, &h->d_hist, (size_t)h->nbins * h->hist_stride));
+    }
     CK(hipMemsetAsync(h->d_hist, 0, sizeof(double) * h->nbins * h->hist_stride, h->st));
     CK(hipMemsetAsync(h->d_done, 0, sizeof(int) * h->nbins, h->st));
     for (int b0 = 0; b0 < h->nbins; b0 += h->group) {
@@ -995,8 +1002,13 @@ int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter) {
     CK(hipStreamSynchronize(h->st));
     prof_flush(h);
     h->cg_ready = true;
-    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    // The captured iteration bakes in the loop-control scalars and the history
+    // buffer; keep it while those stay the same.
+    if (h->gexec && (h->g_thresh != thresh || h->g_max_iter != max_iter || h->g_hist != h->d_hist)) {
+        (void)hipGraphExecDestroy(h->gexec);
+        h->gexec = nullptr;
+        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    }
     return CGS_OK;
 }
 
@@ -1042,6 +1054,9 @@ int cgs_cg_iterate(cgs_ctx* h, int mode, int n_iter) {
                 CK(e);
                 h->graph = g;
                 CK(hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0));
+                h->g_thresh = h->thresh;
+                h->g_max_iter = h->max_iter;
+                h->g_hist = h->d_hist;
             }
             for (int it = 0; it < n_iter; ++it) CK(hipGraphLaunch(h->gexec, h->st));
         }
@@ -1088,9 +1103,12 @@ int cgs_cg_sense_images(cgs_ctx* h, const float* img, double thresh, int max_ite
     h->group = 1;
     h->thresh = thresh;
     h->max_iter = max_iter;
-    dfree(h, h->d_hist, 1);
-    h->hist_stride = max_iter + 1;
-    CKS(dalloc(h, &h->d_hist, (size_t)h->hist_stride));
+    if (!h->d_hist || h->hist_stride != max_iter + 1 || h->hist_bins != 1) {
+        dfree(h, h->d_hist, 1);
+        h->hist_stride = max_iter + 1;
+        h->hist_bins = 1;
+        CKS(dalloc(h, &h->d_hist, (size_t)h->hist_stride));
+    }
     CK(hipMemsetAsync(h->d_hist, 0, sizeof(double) * h->hist_stride, h->st));
     CK(hipMemsetAsync(h->d_done, 0, sizeof(int), h->st));
     float2* tmp = nullptr;

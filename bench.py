@@ -1,0 +1,229 @@
+#!/usr/bin/env python
+"""CG-SENSE iterations/s on one MI355X.
+
+Workload (default): the single-frame full-size case -- 160x160x112 image on a
+320x320x224 grid, 16 coils, R=3x2 wave-CAIPI (2987 of 17920 lines x 320
+samples, 6 wave cycles), k-space synthesized from a seeded phantom x smooth
+coil maps + complex noise, FP32 gridding/FFT with FP64 CG scalars.  A step is
+one CG-SENSE iteration: one E^H W E application over all coils plus the CG
+update (Reco::cgSenseDynamic loop body, Reco.cpp:230-311).  Inputs are
+resident in HBM before the timed region; the right-hand side E^H W y is formed
+before it.
+
+The JSON line carries the measured rate, the relative L2 error of the final
+image against the FP64 CPU oracle after the same iterations, the roofline of
+the dominant kernel (HIP-event timing on the library's stream, algorithmic
+bytes per launch) and the CPU baseline (the oracle's FP32 restatement of the
+reference loops, timed on this host for one iteration).
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(name, ng, C, m, vs):
+    """HBM bytes one launch must move at minimum (complex64 = 8 B)."""
+    per = {
+        "fft_z_fwd": (16 * C + 12) * ng,             # p, S_c, kft in; coil grids out
+        "fft_y_fwd": 16 * C * ng, "fft_x_fwd": 16 * C * ng,
+        "fft_x_inv": 16 * C * ng, "fft_y_inv": 16 * C * ng,
+        "fft_z_inv": (16 * C + 20) * ng,             # grids, S_c, kft, p in; q out
+        "degrid": 8 * C * ng + m * (16 + 8 * vs + 4 + 8 * C),
+        "grid": 8 * C * ng + m * (16 + 8 * vs + 8 * C),
+        "cg_update_xr": 48 * ng,
+        "cg_update_p": 24 * ng,
+        "cg_alpha": 0, "cg_beta": 0,
+    }
+    return per.get(name, 0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="single_160x160x112")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle accuracy check")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--tile", type=int, default=-1, help="gridding tile config (-1: autotune)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(bins shard across GPUs without communication; see DESIGN.md)", file=sys.stderr)
+        sys.exit(1)
+
+    import torch  # device-wide synchronisation around the timed region
+
+    pkg = importlib.import_module("4d_lung_amd")
+    synth = pkg.synth
+    cfg = synth.CONFIGS[args.workload]
+    if cfg["bins"] != 1:
+        raise SystemExit("bench.py times single-frame workloads; use bench_4d.py for binned ones")
+    grid, samp, image, C = cfg["grid"], cfg["samp"], cfg["image"], cfg["coils"]
+    ng, ns = int(np.prod(grid)), int(np.prod(samp))
+    K, W = args.steps, args.warmup
+
+    t0 = time.time()
+    traj = synth.wave_caipi_trajectory(grid, samp, cfg["cycles"], cfg["amp"])
+    lines = synth.caipi_mask(samp[1], samp[2], cfg["R"], cfg["shift"])
+    smask = synth.sample_mask_from_lines(lines, samp[0])
+    maps = synth.coil_maps(grid, C, cfg["seed"] + 1)
+    rho = synth.phantom(grid, image, cfg["seed"] + 2).reshape(-1)
+    log("synthetic inputs %.1fs (%d lines acquired of %d)" % (time.time() - t0, lines.sum(), lines.size))
+
+    ctx = pkg.Context(grid, samp, C)
+    ctx.set_trajectory(traj)
+    ctx.calculate_kernel_ft()
+    ctx.set_maps(maps)
+    # k-space of the phantom: E applied on the acquired samples, then noise
+    mask_data = np.zeros((1, ns), np.complex64)
+    mask_data[0, smask] = 1
+    ctx.exclude_missing_data(mask_data)
+    img = maps * rho[None, :]
+    y = ctx.nufft_fwd(img)
+    del img
+    data = synth.add_noise(y, smask, 1e-3, cfg["seed"] + 3)
+    del y
+    ctx.exclude_missing_data(data)
+    ctx.set_bin_data(0, data)
+    m = ctx.bin_count(0)
+    kw = ctx.kernel_info()[0]
+    vs = (3 * kw + 1) & ~1
+    log("data synthesized, %d active samples per coil, %.1fs" % (m, time.time() - t0))
+
+    # gridding tile autotune: fastest of the shapes on one E^H W E application
+    if args.tile < 0:
+        best = None
+        p = np.ascontiguousarray(rho.astype(np.complex64))
+        for t in range(pkg.cgs.lib().cgs_num_tile_configs()):
+            ctx.set_tile_config(t)
+            ctx.set_bin_data(0, data)
+            ctx.apply_normal(p)
+            ctx.profile(True)
+            ctx.profile_reset()
+            ctx.apply_normal(p)
+            rep = ctx.profile_report()
+            ctx.profile(False)
+            ms = rep["grid"][1] + rep["degrid"][1]
+            log("tile config %d %s: grid+degrid %.3f ms" % (t, ctx.tile_config()[1], ms))
+            if best is None or ms < best[1]:
+                best = (t, ms)
+        tcfg = best[0]
+    else:
+        tcfg = args.tile
+    ctx.set_tile_config(tcfg)
+    ctx.set_bin_data(0, data)
+
+    # warm-up (also instantiates the captured iteration graph)
+    ctx.cg_prepare(0.0, K - 1)
+    ctx.cg_iterate(pkg.SOLVE_BATCHED, max(1, W))
+    # timed region: exactly K iterations
+    ctx.cg_prepare(0.0, K - 1)
+    torch.cuda.synchronize()
+    ts = time.perf_counter()
+    ctx.cg_iterate(pkg.SOLVE_BATCHED, K)
+    torch.cuda.synchronize()
+    te = time.perf_counter()
+    x, res, its = ctx.cg_results()
+    assert int(its[0]) == K, its
+    secs = te - ts
+    value = K / secs
+    log("timed %d iterations: %.2f ms/iter, %.1f it/s" % (K, 1e3 * secs / K, value))
+
+    # plain single-frame path (host reads the residual every iteration)
+    ctx.cg_prepare(0.0, K - 1)
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    ctx.cg_iterate(pkg.SOLVE_PLAIN, K)
+    torch.cuda.synchronize()
+    plain_value = K / (time.perf_counter() - tp)
+
+    # per-kernel HIP-event timing over 3 profiled iterations
+    ctx.cg_prepare(0.0, K - 1)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.cg_iterate(pkg.SOLVE_BATCHED, 3)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    kernels = {}
+    for name, (cnt, ms) in rep.items():
+        avg = ms / cnt
+        b = algorithmic_bytes(name, ng, C, m, vs)
+        kernels[name] = dict(count=cnt, avg_ms=avg, gbs=(b / (avg * 1e-3) / 1e9) if b else None, bytes=b)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count"])
+    dk = kernels[dom]
+    for name in sorted(kernels, key=lambda k: -kernels[k]["avg_ms"]):
+        kk = kernels[name]
+        log("  %-14s %8.3f ms  %s" % (name, kk["avg_ms"], "%.0f GB/s" % kk["gbs"] if kk["gbs"] else ""))
+
+    out = {
+        "metric": "CG-SENSE iterations/sec (E^H E applies/s) at ≤1e-4 rel-L2, 1 MI355X vs plain",
+        "value": value,
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": 1e3 * secs / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 CG scalars)",
+        "data": "synthetic (seeded phantom x smooth coil maps + noise, wave-CAIPI)",
+        "config": {"workload": args.workload, "image": list(image), "grid": list(grid), "coils": C,
+                   "samples_per_coil": int(m), "acceleration": list(cfg["R"]), "cg_iterations": K,
+                   "gridding_tile": list(ctx.tile_config()[1])},
+        "plain_path_value": plain_value,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (dk["gbs"] / HBM_PEAK_GBS) if dk["gbs"] else None, "traffic": None,
+                     "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"]},
+        "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "gbs": round(v["gbs"], 1) if v["gbs"] else None}
+                    for k, v in kernels.items()},
+    }
+    del ctx
+
+    from oracle import oracle as orc
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if not args.no_check:
+        tc = time.time()
+        o = orc.OracleReco(grid, samp, C, nthreads=threads)
+        o.set_trajectory(traj)
+        o.calculate_weights(data)
+        o.calculate_kernel_ft()
+        o.set_maps(maps)
+        xr, resr, itr = o.cg_sense(data, 0.0, K - 1, prec=64)
+        err = float(np.linalg.norm(x[0].ravel() - xr.ravel()) / np.linalg.norm(xr.ravel()))
+        out["rel_l2_vs_fp64_oracle"] = err
+        out["within_tolerance"] = bool(err <= 1e-4)
+        log("FP64 oracle check: rel-L2 %.3e (%.0fs)" % (err, time.time() - tc))
+        del o
+    if not args.no_cpu:
+        o = orc.OracleReco(grid, samp, C, nthreads=threads)
+        o.set_trajectory(traj)
+        o.calculate_weights(data)
+        o.calculate_kernel_ft()
+        o.set_maps(maps)
+        _, _, itc, loop_s = o.cg_sense(data, 0.0, 0, prec=32, timed=True)
+        out["cpu_baseline"] = {"value": itc / loop_s, "unit": "iterations/s", "cores": threads, "kind": "port",
+                               "sample": "%d CG iteration(s) of the same workload, FP32 restatement of "
+                                         "Reco.cpp:175-553 (coil-parallel OpenMP, own CPU FFT)" % itc}
+        log("CPU baseline: %.3f it/s on %d threads" % (itc / loop_s, threads))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -95,7 +95,6 @@ struct Plan1D {
 
 std::vector<int> factorize(int n) {
     std::vector<int> f;
-    while (n % 8 == 0) { f.push_back(8); n /= 8; }
     while (n % 4 == 0) { f.push_back(4); n /= 4; }
     while (n % 2 == 0) { f.push_back(2); n /= 2; }
     for (int p = 3; n > 1; p += 2) {
@@ -119,6 +118,44 @@ Plan1D<T> make_plan(int n) {
     return p;
 }
 
+// In-register DFT of length R (sign as in fft1); radix 2 and 4 written out,
+// other radices by the definition with the plan's root table.
+template <typename T>
+inline void small_dft(const Plan1D<T>& P, int R, int sign, Cplx<T>* v, Cplx<T>* y) {
+    if (R == 2) {
+        y[0] = {v[0].re + v[1].re, v[0].im + v[1].im};
+        y[1] = {v[0].re - v[1].re, v[0].im - v[1].im};
+        return;
+    }
+    if (R == 4) {
+        const Cplx<T> a0 = {v[0].re + v[2].re, v[0].im + v[2].im};
+        const Cplx<T> a1 = {v[0].re - v[2].re, v[0].im - v[2].im};
+        const Cplx<T> a2 = {v[1].re + v[3].re, v[1].im + v[3].im};
+        const Cplx<T> d = {v[1].re - v[3].re, v[1].im - v[3].im};
+        // d * (-i) for the forward sign, d * (+i) for the backward sign
+        const Cplx<T> a3 = sign < 0 ? Cplx<T>{d.im, -d.re} : Cplx<T>{-d.im, d.re};
+        y[0] = {a0.re + a2.re, a0.im + a2.im};
+        y[2] = {a0.re - a2.re, a0.im - a2.im};
+        y[1] = {a1.re + a3.re, a1.im + a3.im};
+        y[3] = {a1.re - a3.re, a1.im - a3.im};
+        return;
+    }
+    const int rstride = P.n / R;   // exp(-2 pi i u/R) = tw[u * n/R]
+    for (int u = 0; u < R; ++u) {
+        T sr = 0, si = 0;
+        int t = 0;
+        for (int i = 0; i < R; ++i) {
+            const T wr = P.tr[t * rstride];
+            const T wi = sign < 0 ? P.ti[t * rstride] : -P.ti[t * rstride];
+            sr += v[i].re * wr - v[i].im * wi;
+            si += v[i].re * wi + v[i].im * wr;
+            t += u;
+            if (t >= R) t -= R;
+        }
+        y[u] = {sr, si};
+    }
+}
+
 template <typename T>
 void fft1(const Plan1D<T>& P, Cplx<T>* x, Cplx<T>* work, int sign) {
     const int n = P.n;
@@ -126,11 +163,10 @@ void fft1(const Plan1D<T>& P, Cplx<T>* x, Cplx<T>* work, int sign) {
     Cplx<T>* in = x;
     Cplx<T>* out = work;
     int Ls = 1;
-    Cplx<T> v[64];
+    Cplx<T> v[64], y[64];
     for (int R : P.rad) {
         const int m = n / R;
         const int tstride = n / (Ls * R);
-        const int rstride = n / R;        // exp(-2 pi i u/R) = tw[u * n/R]
         for (int j = 0; j < m; ++j) {
             const int k = j % Ls;
             for (int i = 0; i < R; ++i) {
@@ -145,19 +181,9 @@ void fft1(const Plan1D<T>& P, Cplx<T>* x, Cplx<T>* work, int sign) {
                 }
                 v[i] = a;
             }
+            small_dft(P, R, sign, v, y);
             const int base = (j / Ls) * Ls * R + k;
-            for (int u = 0; u < R; ++u) {
-                T sr = 0, si = 0;
-                for (int i = 0; i < R; ++i) {
-                    const int t = ((i * u) % R) * rstride;
-                    const T wr = P.tr[t];
-                    const T wi = sign < 0 ? P.ti[t] : -P.ti[t];
-                    sr += v[i].re * wr - v[i].im * wi;
-                    si += v[i].re * wi + v[i].im * wr;
-                }
-                out[base + u * Ls].re = sr;
-                out[base + u * Ls].im = si;
-            }
+            for (int u = 0; u < R; ++u) out[base + u * Ls] = y[u];
         }
         std::swap(in, out);
         Ls *= R;
@@ -184,31 +210,39 @@ void sign_alternate3d(Cplx<T>* a, const int* N) {
 }
 
 // Unnormalized 3D DFT along all three axes of an [x][y][z] (z fastest) array.
+// Strided axes are transformed in blocks of ZB neighbouring lines so every
+// gather reads contiguous runs.
 template <typename T>
 void fft3_raw(Cplx<T>* a, const int* N, int sign) {
     const int nx = N[0], ny = N[1], nz = N[2];
     Plan1D<T> px = make_plan<T>(nx), py = make_plan<T>(ny), pz = make_plan<T>(nz);
     const int maxn = std::max(nx, std::max(ny, nz));
-    std::vector<Cplx<T>> line(maxn), work(maxn);
+    constexpr int ZB = 16;
+    std::vector<Cplx<T>> blk((size_t)ZB * maxn), work(maxn);
     // z lines (contiguous)
     for (long xy = 0; xy < (long)nx * ny; ++xy) fft1(pz, a + xy * nz, work.data(), sign);
     // y lines
     for (int x = 0; x < nx; ++x)
-        for (int z = 0; z < nz; ++z) {
-            Cplx<T>* base = a + (long)x * ny * nz + z;
-            for (int y = 0; y < ny; ++y) line[y] = base[(long)y * nz];
-            fft1(py, line.data(), work.data(), sign);
-            for (int y = 0; y < ny; ++y) base[(long)y * nz] = line[y];
+        for (int z0 = 0; z0 < nz; z0 += ZB) {
+            const int zb = std::min(ZB, nz - z0);
+            Cplx<T>* base = a + (long)x * ny * nz + z0;
+            for (int y = 0; y < ny; ++y)
+                for (int j = 0; j < zb; ++j) blk[(size_t)j * ny + y] = base[(long)y * nz + j];
+            for (int j = 0; j < zb; ++j) fft1(py, blk.data() + (size_t)j * ny, work.data(), sign);
+            for (int y = 0; y < ny; ++y)
+                for (int j = 0; j < zb; ++j) base[(long)y * nz + j] = blk[(size_t)j * ny + y];
         }
     // x lines
-    for (int y = 0; y < ny; ++y)
-        for (int z = 0; z < nz; ++z) {
-            Cplx<T>* base = a + (long)y * nz + z;
-            const long st = (long)ny * nz;
-            for (int x = 0; x < nx; ++x) line[x] = base[x * st];
-            fft1(px, line.data(), work.data(), sign);
-            for (int x = 0; x < nx; ++x) base[x * st] = line[x];
-        }
+    const long st = (long)ny * nz;
+    for (long yz0 = 0; yz0 < st; yz0 += ZB) {
+        const int zb = (int)std::min<long>(ZB, st - yz0);
+        Cplx<T>* base = a + yz0;
+        for (int x = 0; x < nx; ++x)
+            for (int j = 0; j < zb; ++j) blk[(size_t)j * nx + x] = base[x * st + j];
+        for (int j = 0; j < zb; ++j) fft1(px, blk.data() + (size_t)j * nx, work.data(), sign);
+        for (int x = 0; x < nx; ++x)
+            for (int j = 0; j < zb; ++j) base[x * st + j] = blk[(size_t)j * nx + x];
+    }
 }
 
 // Fourier::transformFWD (Fourier.cpp:19-24): sign-alternate, forward DFT, sign-alternate.
@@ -689,7 +723,7 @@ int apply_normal(Oracle& o, const Cplx<T>* p, Cplx<T>* q, std::vector<Cplx<T>>& 
 // Reco::cgSenseDynamic (Reco.cpp:175-321) from the coil images img_c = E^H W y.
 template <typename T>
 int cg_sense(Oracle& o, std::vector<Cplx<T>>& img, double thresh, int max_iter, Cplx<T>* x,
-             double* resid_out, int* iters_out) {
+             double* resid_out, int* iters_out, double* loop_seconds = nullptr) {
     const long Ng = o.Ng;
     const int nt = o.nthreads;
     std::vector<Cplx<T>> a(Ng), q(Ng), p(Ng), r(Ng);
@@ -708,6 +742,7 @@ int cg_sense(Oracle& o, std::vector<Cplx<T>>& img, double thresh, int max_iter, 
     for (long i = 0; i < Ng; i++) { p[i] = a[i]; r[i] = a[i]; x[i].re = 0; x[i].im = 0; q[i].re = 0; q[i].im = 0; }
     int iteration = 0;
     double current = 1000;
+    const double t0 = omp_get_wtime();
     while (current > thresh) {
         const int st = apply_normal(o, p.data(), q.data(), img, data);
         if (st) { *iters_out = iteration; return st; }
@@ -745,6 +780,7 @@ int cg_sense(Oracle& o, std::vector<Cplx<T>>& img, double thresh, int max_iter, 
         iteration++;
         if (iteration > max_iter) break;
     }
+    if (loop_seconds) *loop_seconds = omp_get_wtime() - t0;
     *iters_out = iteration;
     return 0;
 }
@@ -770,8 +806,8 @@ int orc_apply_normal(void* h, int prec, const void* p, void* q) {
 // data: C x nS complex64 (memory order).  x_out: Ng complex of prec.
 // resid_out: at least max_iter+1 doubles.  max_iter is the reference's
 // nMaxIter (Reco.cpp:228): at most max_iter+1 passes are made.
-int orc_cg_sense(void* h, int prec, const float* data, double thresh, int max_iter, void* x_out,
-                 double* resid_out, int* iters_out) {
+int orc_cg_sense_timed(void* h, int prec, const float* data, double thresh, int max_iter, void* x_out,
+                       double* resid_out, int* iters_out, double* loop_seconds) {
     Oracle* o = (Oracle*)h;
     if (o->kft.empty() || o->maps.empty()) return 2;
     const size_t nd = (size_t)o->C * o->nS;
@@ -781,11 +817,20 @@ int orc_cg_sense(void* h, int prec, const float* data, double thresh, int max_it
         std::vector<Cplx<double>> d(nd), img((size_t)o->C * o->Ng);
         for (size_t i = 0; i < nd; i++) { d[i].re = dc[2 * i]; d[i].im = dc[2 * i + 1]; }
         orc_nufft_bwd(h, 64, d.data(), img.data());
-        return cg_sense(*o, img, thresh, max_iter, (Cplx<double>*)x_out, resid_out, iters_out);
+        return cg_sense(*o, img, thresh, max_iter, (Cplx<double>*)x_out, resid_out, iters_out, loop_seconds);
     }
     std::vector<Cplx<float>> img((size_t)o->C * o->Ng);
     orc_nufft_bwd(h, 32, dc.data(), img.data());
-    return cg_sense(*o, img, thresh, max_iter, (Cplx<float>*)x_out, resid_out, iters_out);
+    return cg_sense(*o, img, thresh, max_iter, (Cplx<float>*)x_out, resid_out, iters_out, loop_seconds);
 }
+
+int orc_cg_sense(void* h, int prec, const float* data, double thresh, int max_iter, void* x_out,
+                 double* resid_out, int* iters_out) {
+    return orc_cg_sense_timed(h, prec, data, thresh, max_iter, x_out, resid_out, iters_out, nullptr);
+}
+
+// std::cyl_bessel_i(0, x) of libstdc++ (C++17 special functions), a second
+// independent I0 for the tests.
+double orc_i0_std(double x) { return std::cyl_bessel_i(0.0, x); }
 
 }  // extern "C"

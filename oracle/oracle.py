@@ -64,6 +64,11 @@ def lib():
         L.orc_apply_normal.argtypes = [P, ctypes.c_int, P, P]
         L.orc_cg_sense.restype = ctypes.c_int
         L.orc_cg_sense.argtypes = [P, ctypes.c_int, P, ctypes.c_double, ctypes.c_int, P, P, P]
+        L.orc_cg_sense_timed.restype = ctypes.c_int
+        L.orc_cg_sense_timed.argtypes = [P, ctypes.c_int, P, ctypes.c_double, ctypes.c_int, P, P, P,
+                                         ctypes.POINTER(ctypes.c_double)]
+        L.orc_i0_std.restype = ctypes.c_double
+        L.orc_i0_std.argtypes = [ctypes.c_double]
         _lib = L
     return _lib
 
@@ -78,6 +83,10 @@ def _ptr(a):
 
 def i0(x):
     return lib().orc_i0(float(x))
+
+
+def i0_std(x):
+    return lib().orc_i0_std(float(x))
 
 
 def kb_axis(dxf, n, w, b):
@@ -198,14 +207,18 @@ class OracleReco:
             raise RuntimeError("oracle apply_normal status %d" % st)
         return q
 
-    def cg_sense(self, data, thresh=0.0, max_iter=15, prec=64):
+    def cg_sense(self, data, thresh=0.0, max_iter=15, prec=64, timed=False):
+        """Reco::cgSenseDynamic after density correction and nufftBWD of data.
+        With timed=True also returns the seconds spent in the iteration loop."""
         dt = np.complex128 if prec == 64 else np.complex64
         d = np.ascontiguousarray(data, dtype=np.complex64).reshape(self.ncoil, self.ns)
         x = np.zeros(self.ng, dt)
         res = np.zeros(max_iter + 2, np.float64)
         it = ctypes.c_int(0)
-        st = lib().orc_cg_sense(self.h, prec, _ptr(d), float(thresh), int(max_iter), _ptr(x), _ptr(res),
-                                ctypes.byref(it))
+        secs = ctypes.c_double(0.0)
+        st = lib().orc_cg_sense_timed(self.h, prec, _ptr(d), float(thresh), int(max_iter), _ptr(x), _ptr(res),
+                                      ctypes.byref(it), ctypes.byref(secs))
         if st:
             raise RuntimeError("oracle cg_sense status %d" % st)
-        return x.reshape(self.grid), res[: it.value], it.value
+        out = (x.reshape(self.grid), res[: it.value], it.value)
+        return out + (secs.value,) if timed else out

@@ -24,6 +24,7 @@ def test_i0_matches_scipy(orc):
     xs = np.concatenate([np.linspace(0, 16, 2001), [0.0, 1e-3, 14.9086, 20.0, 30.0]])
     got = np.array([orc.i0(x) for x in xs])
     np.testing.assert_allclose(got, sp.i0(xs), rtol=4e-15, atol=0)
+    np.testing.assert_allclose(got, [orc.i0_std(x) for x in xs], rtol=4e-15, atol=0)
     assert orc.i0(0.0) == 1.0
 
 
