@@ -34,6 +34,7 @@ SOURCES = [
     ("cgs_nufft.hip", []),
     ("cgs_cg.hip", []),
     ("cgs_setup.hip", ["-ffp-contract=off"]),
+    ("cgs_voronoi.hip", ["-ffp-contract=off"]),
     ("cgs_api.hip", ["-ffp-contract=off"]),
 ]
 

@@ -82,6 +82,9 @@ def lib():
         "cgs_bin_count": (I, [P, I, ctypes.POINTER(LL)]),
         "cgs_bin_tables": (I, [P, I, P, P, P, P, P]),
         "cgs_kernel_info": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(D)]),
+        "cgs_voronoi_volumes": (I, [P, LL, P]),
+        "cgs_voronoi_density": (I, [P, LL, I, I, D, P]),
+        "cgs_set_bin_voronoi": (I, [P, I]),
         "cgs_fft_create": (I, [I3, ctypes.POINTER(P)]),
         "cgs_fft_destroy": (I, [P]),
         "cgs_fft_forward": (I, [P, P]),
@@ -269,6 +272,10 @@ class Context:
         self.cg_iterate(mode, max_iter + 1)
         return self.cg_results()
 
+    def set_bin_voronoi(self, b):
+        """Voronoi weights of the bin's active samples become its DCF."""
+        self._ck(self._L.cgs_set_bin_voronoi(self.h, int(b)))
+
     # -- introspection
     def bin_count(self, b=0):
         m = ctypes.c_longlong()
@@ -310,6 +317,26 @@ class Context:
 
     def device_bytes(self):
         return int(self._L.cgs_device_bytes(self.h))
+
+
+def voronoi_volumes(points):
+    """Voronoi cell volumes on the device (0 for unbounded cells)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    vol = np.zeros(len(p), np.float64)
+    st = lib().cgs_voronoi_volumes(_ptr(p), len(p), _ptr(vol))
+    if st:
+        raise CgsError(st, "cgs_voronoi_volumes failed")
+    return vol
+
+
+def voronoi_density(points, average=False, sx=0, cutoff=1.0):
+    """estimateDensityV3D (functions.h:161-258) on the device."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    d = np.zeros(len(p), np.float32)
+    st = lib().cgs_voronoi_density(_ptr(p), len(p), int(bool(average)), int(sx), float(cutoff), _ptr(d))
+    if st:
+        raise CgsError(st, "cgs_voronoi_density failed")
+    return d
 
 
 class Fourier:

@@ -696,6 +696,7 @@ int cgs_destroy(cgs_ctx* h) {
 int cgs_set_kernel(cgs_ctx* h, float w, float b, int order) {
     if (!h) return CGS_ERR_ARG;
     if (!(w > 0.0f) || order < 1) return fail(h, CGS_ERR_ARG, "invalid kernel parameters");
+    if (h->kb_ready && w == h->w && b == h->b && order == h->order) return CGS_OK;   // nothing changes
     h->w = w; h->b = b; h->order = order;
     h->kb_ready = false;
     h->kft_ready = false;
@@ -1203,6 +1204,126 @@ int cgs_profile_report(cgs_ctx* h, char* buf, int len) {
 }
 
 long long cgs_device_bytes(const cgs_ctx* h) { return h ? h->dev_bytes : 0; }
+
+// ------------------------------------------------------------------ Voronoi
+static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, double* vol_host, hipStream_t st) {
+    if (np < 2) {
+        for (long long i = 0; i < np; i++) vol_host[i] = 0.0;
+        return CGS_OK;
+    }
+    // uniform hash: about six points per cell, ascending point index per cell
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; a++) { lo[a] = pts[a]; hi[a] = pts[a]; }
+    for (long long i = 0; i < np; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::fmin(lo[a], pts[3 * i + a]);
+            hi[a] = std::fmax(hi[a], pts[3 * i + a]);
+        }
+    double vol = 1.0;
+    for (int a = 0; a < 3; a++) vol *= std::fmax(hi[a] - lo[a], 1e-12);
+    VorHash H{};
+    H.h = std::cbrt(vol * 6.0 / (double)np);
+    for (int a = 0; a < 3; a++) { H.lo[a] = lo[a]; H.n[a] = (int)((hi[a] - lo[a]) / H.h) + 1; }
+    const long long nc = (long long)H.n[0] * H.n[1] * H.n[2];
+    std::vector<int> start(nc + 1, 0), key(np), idx(np);
+    for (long long i = 0; i < np; i++) {
+        int c3[3];
+        for (int a = 0; a < 3; a++) {
+            int k = (int)((pts[3 * i + a] - H.lo[a]) / H.h);
+            c3[a] = k < 0 ? 0 : (k >= H.n[a] ? H.n[a] - 1 : k);
+        }
+        key[i] = (c3[0] * H.n[1] + c3[1]) * H.n[2] + c3[2];
+        start[key[i] + 1]++;
+    }
+    for (long long c = 0; c < nc; c++) start[c + 1] += start[c];
+    std::vector<int> cur(start.begin(), start.end() - 1);
+    for (long long i = 0; i < np; i++) idx[cur[key[i]]++] = (int)i;
+    double ext = 0.0;
+    for (int a = 0; a < 3; a++) ext = std::fmax(ext, H.h * H.n[a]);
+    const double r0 = 4.0 * ext;
+
+    double *dp = nullptr, *dv = nullptr;
+    int *ds = nullptr, *di = nullptr, *dov = nullptr;
+    CKS(dalloc(h, &dp, (size_t)np * 3));
+    CKS(dalloc(h, &dv, (size_t)np));
+    CKS(dalloc(h, &ds, (size_t)nc + 1));
+    CKS(dalloc(h, &di, (size_t)np));
+    CKS(dalloc(h, &dov, 1));
+    CK(hipMemcpy(dp, pts, sizeof(double) * 3 * np, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ds, start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
+    CK(hipMemcpy(di, idx.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    CK(hipMemset(dov, 0, sizeof(int)));
+    H.start = ds;
+    H.idx = di;
+    CK(launch_voronoi(dp, np, H, r0, dv, dov, st));
+    int over = 0;
+    CK(hipMemcpyAsync(vol_host, dv, sizeof(double) * np, hipMemcpyDeviceToHost, st));
+    CK(hipMemcpyAsync(&over, dov, sizeof(int), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    dfree(h, dp, (size_t)np * 3); dfree(h, dv, (size_t)np); dfree(h, ds, (size_t)nc + 1);
+    dfree(h, di, (size_t)np); dfree(h, dov, 1);
+    if (over) return fail(h, CGS_ERR_ARG, std::to_string(over) + " Voronoi cells exceeded the face/vertex limits");
+    return CGS_OK;
+}
+
+// functions.h:212-245: optional averaging over i mod sx, then / max.
+static void normalise_density(const double* vol, long long np, int average, int sx, double cutoff, float* dcf) {
+    std::vector<float> cv(np);
+    for (long long i = 0; i < np; i++) cv[i] = (float)vol[i];
+    if (average && sx > 0) {
+        std::vector<float> avg(sx, 0.0f);
+        for (long long i = 0; i < np; i++) avg[i % sx] += cv[i];
+        double tot = 0.0;
+        for (int i = 0; i < sx; i++) tot += avg[i];
+        tot = 1.0 / tot;
+        const int cut = (int)((sx - 1) * cutoff);
+        for (long long i = 0; i < np; i++) cv[i] = (float)(tot * avg[std::min((int)(i % sx), cut)]);
+    }
+    double mx = 0.0;
+    for (long long i = 0; i < np; i++)
+        if (cv[i] > mx) mx = cv[i];
+    mx = 1.0 / mx;
+    for (long long i = 0; i < np; i++) dcf[i] = (float)(cv[i] * mx);
+}
+
+int cgs_voronoi_volumes(const double* pts, long long np, double* vol) {
+    if (!pts || !vol || np < 0) return CGS_ERR_ARG;
+    return voronoi_volumes_dev(nullptr, pts, np, vol, nullptr);
+}
+
+int cgs_voronoi_density(const double* pts, long long np, int average, int sx, double cutoff, float* dcf) {
+    if (!pts || !dcf || np < 1) return CGS_ERR_ARG;
+    std::vector<double> vol(np);
+    const int st = voronoi_volumes_dev(nullptr, pts, np, vol.data(), nullptr);
+    if (st != CGS_OK) return st;
+    normalise_density(vol.data(), np, average, sx, cutoff, dcf);
+    return CGS_OK;
+}
+
+int cgs_set_bin_voronoi(cgs_ctx* h, int bin) {
+    if (!h || bin < 0 || bin >= h->nbins) return CGS_ERR_ARG;
+    BinData& B = h->bins[bin];
+    if (!B.built) return fail(h, CGS_ERR_STATE, "bin has no samples");
+    std::vector<int> perm(B.m);
+    std::vector<double> traj((size_t)h->ns * 3);
+    if (B.m) CK(hipMemcpy(perm.data(), B.perm, sizeof(int) * B.m, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(traj.data(), h->d_traj, sizeof(double) * 3 * h->ns, hipMemcpyDeviceToHost));
+    // the bin's active samples in original index order
+    std::vector<int> act(perm);
+    std::sort(act.begin(), act.end());
+    std::vector<double> pts((size_t)B.m * 3);
+    for (long long s = 0; s < B.m; s++)
+        for (int a = 0; a < 3; a++) pts[3 * s + a] = traj[3 * (size_t)act[s] + a];
+    std::vector<double> vol(B.m);
+    CKS(voronoi_volumes_dev(h, pts.data(), B.m, vol.data(), h->st));
+    std::vector<float> w(B.m), full(h->ns, 0.0f);
+    normalise_density(vol.data(), B.m, 0, 0, 1.0, w.data());
+    for (long long s = 0; s < B.m; s++) full[act[s]] = w[s];
+    if (!B.dcf_full) CKS(dalloc(h, &B.dcf_full, h->ns));
+    CK(hipMemcpy(B.dcf_full, full.data(), sizeof(float) * h->ns, hipMemcpyHostToDevice));
+    free_packed(h);
+    return CGS_OK;
+}
 
 // ------------------------------------------------------------------ Fourier
 int cgs_fft_create(const int dims[3], cgs_fft** out) {

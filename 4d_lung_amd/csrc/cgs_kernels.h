@@ -138,4 +138,18 @@ hipError_t launch_fill_zero(float2* a, long long n, hipStream_t s);
 hipError_t launch_coil_combine(const float2* G, const float2* maps, float2* q, long long ng, int C, hipStream_t s);
 hipError_t launch_scale_d(float2* a, long long n, double f, hipStream_t s);
 
+// ---------------------------------------------------------------- Voronoi
+// Uniform hash of the points: cell (x, y, z) holds idx[start[c] .. start[c+1])
+// in ascending point order, c = (x*n1 + y)*n2 + z.
+struct VorHash {
+    double lo[3];
+    double h;
+    int n[3];
+    const int* start;
+    const int* idx;
+};
+
+hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double r0, double* vol, int* overflow,
+                          hipStream_t s);
+
 }  // namespace cgs

@@ -197,6 +197,7 @@ def main():
 
     from oracle import oracle as orc
 
+    orc.set_verbose(1)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     if not args.no_check:
         tc = time.time()

@@ -126,6 +126,17 @@ int cgs_bin_tables(cgs_ctx* h, int bin, int32_t* nearest, int32_t* perm, int32_t
                    double* vals);
 int cgs_kernel_info(const cgs_ctx* h, int* kw, int* ntiles, double* thr);
 
+/* ---- Voronoi density compensation (estimateDensityV3D, functions.h:161-258) ---- */
+/* Cell volumes of np points (np x 3 doubles), 0 for unbounded (hull) cells. */
+int cgs_voronoi_volumes(const double* pts, long long np, double* vol);
+/* Weights = volume / max volume (functions.h:234-245); with average != 0 the
+ * per-index averaging of functions.h:212-232 (index i mod sx, cut-off) is
+ * applied first. */
+int cgs_voronoi_density(const double* pts, long long np, int average, int sx, double cutoff, float* dcf);
+/* Voronoi weights over the active samples of one bin become that bin's density
+ * weights (per-bin DCF of the 4D workloads). */
+int cgs_set_bin_voronoi(cgs_ctx* h, int bin);
+
 /* ---- Fourier (Fourier.h:12-54) on one complex grid ---- */
 int cgs_fft_create(const int dims[3], cgs_fft** out);
 int cgs_fft_destroy(cgs_fft* f);

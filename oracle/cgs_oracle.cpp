@@ -32,6 +32,8 @@
 
 namespace {
 
+int g_verbose = 0;   // print one line per CG iteration to stderr
+
 // ---------------------------------------------------------------------------
 // Modified Bessel function I0 -- power series (Abramowitz & Stegun 9.6.12).
 // Stands in for boost::math::cyl_bessel_i(0, x) used at Reco.h:238-240.
@@ -505,6 +507,8 @@ extern "C" {
 
 double orc_i0(double x) { return i0_series(x); }
 
+void orc_set_verbose(int v) { g_verbose = v; }
+
 double orc_kb_axis(float dxf, int N, float w, float b) { return kb_axis_value(dxf, N, w, b); }
 
 float orc_conv_function(const double* dist, const int* N, float w, float b) {
@@ -767,6 +771,7 @@ int cg_sense(Oracle& o, std::vector<Cplx<T>>& img, double thresh, int max_iter, 
         for (long i = 0; i < Ng; i++) current += (T)(r[i].re * r[i].re + r[i].im * r[i].im);
         current = std::sqrt(current);
         if (resid_out) resid_out[iteration] = current;
+        if (g_verbose) std::fprintf(stderr, "[oracle] CG iteration %d residual %.6e\n", iteration, current);
         double rn[2], be[2];
         conj_dot(o, r.data(), r.data(), Ng, rn);
         complex_division(rn, rr, be);

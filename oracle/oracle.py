@@ -67,8 +67,13 @@ def lib():
         L.orc_cg_sense_timed.restype = ctypes.c_int
         L.orc_cg_sense_timed.argtypes = [P, ctypes.c_int, P, ctypes.c_double, ctypes.c_int, P, P, P,
                                          ctypes.POINTER(ctypes.c_double)]
+        L.orc_set_verbose.argtypes = [ctypes.c_int]
         L.orc_i0_std.restype = ctypes.c_double
         L.orc_i0_std.argtypes = [ctypes.c_double]
+        L.orc_voronoi_volumes.restype = ctypes.c_long
+        L.orc_voronoi_volumes.argtypes = [P, ctypes.c_long, P, ctypes.c_int]
+        L.orc_voronoi_density.restype = ctypes.c_long
+        L.orc_voronoi_density.argtypes = [P, ctypes.c_long, P, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -81,12 +86,35 @@ def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def set_verbose(v):
+    lib().orc_set_verbose(int(v))
+
+
 def i0(x):
     return lib().orc_i0(float(x))
 
 
 def i0_std(x):
     return lib().orc_i0_std(float(x))
+
+
+def voronoi_volumes(points, nthreads=0):
+    """Cell volumes by half-space clipping (the device algorithm on the CPU)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    vol = np.zeros(len(p), np.float64)
+    over = lib().orc_voronoi_volumes(_ptr(p), len(p), _ptr(vol), int(nthreads))
+    if over:
+        raise RuntimeError("%d Voronoi cells exceeded the face/vertex limits" % over)
+    return vol
+
+
+def voronoi_density(points, nthreads=0):
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    d = np.zeros(len(p), np.float32)
+    over = lib().orc_voronoi_density(_ptr(p), len(p), _ptr(d), int(nthreads))
+    if over:
+        raise RuntimeError("%d Voronoi cells exceeded the face/vertex limits" % over)
+    return d
 
 
 def kb_axis(dxf, n, w, b):
