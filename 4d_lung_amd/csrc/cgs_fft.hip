@@ -192,11 +192,12 @@ template <int DIR>
 struct Dft<7, DIR> : DftOdd<7, DIR> {};
 
 // ------------------------------------------------------- Stockham stages
-// LDS tile layout: element (line index l, column col) at lds[l * NCOL + col].
+// LDS tile layout: element (line index l, column col) at lds[l * STRIDE + col]
+// (STRIDE = NCOL, or NCOL + 1 where rows are later reduced by one thread).
 // Stage with radix R on sub-transforms of size LS (Stockham autosort):
 //   butterfly j reads l = j + i*L/R, twiddles by exp(-+2 pi i k i/(LS R)) with
 //   k = j mod LS, and writes l = (j/LS)*LS*R + k + i*LS.
-template <int L, int R, int LS, int DIR, int NCOL>
+template <int L, int R, int LS, int DIR, int NCOL, int STRIDE>
 __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict__ tw) {
     constexpr int NB = NCOL * (L / R);
     constexpr int PER = (NB + CGS_THREADS - 1) / CGS_THREADS;
@@ -210,7 +211,7 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
             const int col = b % NCOL, j = b / NCOL, k = j % LS;
 #pragma unroll
             for (int i = 0; i < R; ++i) {
-                float2 x = lds[(j + i * (L / R)) * NCOL + col];
+                float2 x = lds[(j + i * (L / R)) * STRIDE + col];
                 if (LS > 1 && i > 0) {
                     float2 t = tw[k * i * TS];
                     if (DIR > 0) t.y = -t.y;
@@ -229,18 +230,18 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
             Dft<R, DIR>::run(v[p]);
             const int base = (j / LS) * LS * R + k;
 #pragma unroll
-            for (int i = 0; i < R; ++i) lds[(base + i * LS) * NCOL + col] = v[p][i];
+            for (int i = 0; i < R; ++i) lds[(base + i * LS) * STRIDE + col] = v[p][i];
         }
     }
     __syncthreads();
 }
 
-template <int L, int S, int LS, int DIR, int NCOL>
+template <int L, int S, int LS, int DIR, int NCOL, int STRIDE = NCOL>
 __device__ __forceinline__ void fft_stages(float2* lds, const float2* __restrict__ tw) {
     if constexpr (S < plan_nstages(L)) {
         constexpr int R = plan_radix(L, S);
-        fft_stage<L, R, LS, DIR, NCOL>(lds, tw);
-        fft_stages<L, S + 1, LS * R, DIR, NCOL>(lds, tw);
+        fft_stage<L, R, LS, DIR, NCOL, STRIDE>(lds, tw);
+        fft_stages<L, S + 1, LS * R, DIR, NCOL, STRIDE>(lds, tw);
     }
 }
 
@@ -335,14 +336,17 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_zpre(ZArgs a) {
 // (IFFT 1/sqrt(Ng) and deapodization *1e-3, Reco.cpp:506-515) and cropped to
 // the reduceFOV box (Reco.cpp:626-671, inclusive upper bound).
 //   a.maps != null : q[bin][cell] = sum_c conj(S[cell][c]) * value
-//                    (mutliplyByConjSenseMap + addToSumOfImages, Reco.cpp:107-129),
-//                    and the block's partial <p, q> goes to a.part[blockIdx.x]
+//                    (mutliplyByConjSenseMap + addToSumOfImages, Reco.cpp:107-129):
+//                    the products replace the tile in LDS (rows padded by one
+//                    element) and one thread per z sums its row over the coils;
+//                    the block's partial <p, q> goes to a.part[blockIdx.x]
 //   a.maps == null : G[bin][cell][c] = value (coil images, nufftBWD)
 template <int L, int W>
 __global__ __launch_bounds__(CGS_THREADS) void k_fft_zpost(ZArgs a) {
-    __shared__ float2 lds[L * W];
+    constexpr int LDW = W + 1;
+    constexpr int RPT = (L + CGS_THREADS - 1) / CGS_THREADS;
+    __shared__ float2 lds[L * LDW];
     __shared__ double red[2 * (CGS_THREADS / 64)];
-    constexpr int PERE = (L * W + CGS_THREADS - 1) / CGS_THREADS;
     const long long outer = blockIdx.x;
     const long long bin = outer / a.nxy, xy = outer % a.nxy;
     if (a.done && a.done[bin]) return;
@@ -351,41 +355,45 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_zpost(ZArgs a) {
     const bool xy_in = !a.crop || (gx >= a.lo[0] && gx <= a.hi[0] && gy >= a.lo[1] && gy <= a.hi[1]);
     float2* g = a.g + (bin * a.ng + cell0) * a.C;
     const bool combine = a.maps != nullptr;
-    float2 acc[PERE];
+    float2 acc[RPT];
 #pragma unroll
-    for (int k = 0; k < PERE; ++k) acc[k] = make_float2(0.0f, 0.0f);
+    for (int r = 0; r < RPT; ++r) acc[r] = make_float2(0.0f, 0.0f);
     for (int c0 = 0; c0 < a.C; c0 += W) {
         for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
             const int c = e % W, l = e / W;
             const int coil = c0 + c;
             float2 v = make_float2(0.0f, 0.0f);
             if (coil < a.C) v = g[(long long)l * a.C + coil];
-            lds[e] = alt_sign(v, l);
+            lds[l * LDW + c] = alt_sign(v, l);
         }
         __syncthreads();
-        fft_stages<L, 0, 1, +1, W>(lds, a.tw);
-#pragma unroll
-        for (int k = 0; k < PERE; ++k) {
-            const int e = threadIdx.x + k * CGS_THREADS;
+        fft_stages<L, 0, 1, +1, W, LDW>(lds, a.tw);
+        for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
             const int c = e % W, l = e / W;
             const int coil = c0 + c;
             const long long cell = cell0 + l;
             float2 v = make_float2(0.0f, 0.0f);
-            if (e < L * W && coil < a.C) {
-                v = alt_sign(lds[e], l);
-                const float s = a.kft ? a.kft[cell] * a.scale : a.scale;
-                v = cscale(v, s);
+            if (coil < a.C) {
+                v = alt_sign(lds[l * LDW + c], l);
+                const float sc = a.kft ? a.kft[cell] * a.scale : a.scale;
+                v = cscale(v, sc);
                 if (!xy_in || (a.crop && (l < a.lo[2] || l > a.hi[2]))) v = make_float2(0.0f, 0.0f);
                 if (combine) v = cmulc(a.maps[cell * a.C + coil], v);
                 else g[(long long)l * a.C + coil] = v;
             }
-            if (combine) {
+            if (combine) lds[l * LDW + c] = v;
+        }
+        if (combine) {
+            __syncthreads();
 #pragma unroll
-                for (int off = W / 2; off >= 1; off >>= 1) {
-                    v.x += __shfl_xor(v.x, off, W);
-                    v.y += __shfl_xor(v.y, off, W);
+            for (int r = 0; r < RPT; ++r) {
+                const int l = threadIdx.x + r * CGS_THREADS;
+                if (l < L) {
+                    float2 sum = make_float2(0.0f, 0.0f);
+#pragma unroll
+                    for (int c = 0; c < W; ++c) sum = cadd(sum, lds[l * LDW + c]);
+                    acc[r] = cadd(acc[r], sum);
                 }
-                acc[k] = cadd(acc[k], v);
             }
         }
         __syncthreads();
@@ -393,16 +401,15 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_zpost(ZArgs a) {
     if (!combine) return;
     double pr = 0.0, pi = 0.0;
 #pragma unroll
-    for (int k = 0; k < PERE; ++k) {
-        const int e = threadIdx.x + k * CGS_THREADS;
-        const int c = e % W, l = e / W;
-        if (e < L * W && c == 0) {
+    for (int r = 0; r < RPT; ++r) {
+        const int l = threadIdx.x + r * CGS_THREADS;
+        if (l < L) {
             const long long idx = bin * a.ng + cell0 + l;
-            a.q[idx] = acc[k];
+            a.q[idx] = acc[r];
             if (a.p) {
                 const float2 pv = a.p[idx];
-                pr += (double)pv.x * (double)acc[k].x + (double)pv.y * (double)acc[k].y;
-                pi += (double)pv.x * (double)acc[k].y - (double)pv.y * (double)acc[k].x;
+                pr += (double)pv.x * (double)acc[r].x + (double)pv.y * (double)acc[r].y;
+                pi += (double)pv.x * (double)acc[r].y - (double)pv.y * (double)acc[r].x;
             }
         }
     }
@@ -444,7 +451,7 @@ static inline int plain_w(int L) { return L <= 256 ? 32 : 16; }
 // below 8K complex (64 KB), else the largest power of two that fits.
 int zpass_w(int L, int C) {
     int w = C <= 8 ? 8 : (C <= 16 ? 16 : 32);
-    while (w > 8 && L * w > 8192) w /= 2;
+    while (w > 8 && L * (w + 1) > 8192) w /= 2;
     return w;
 }
 
@@ -492,10 +499,10 @@ static hipError_t zpre_w(int W, const ZArgs& a, long long nouter, hipStream_t s)
     const dim3 grid((unsigned)nouter);
     if (W == 8) {
         k_fft_zpre<L, 8><<<grid, CGS_THREADS, 0, s>>>(a);
-    } else if (W == 16 && L * 16 <= 8192) {
-        if constexpr (L * 16 <= 8192) k_fft_zpre<L, 16><<<grid, CGS_THREADS, 0, s>>>(a);
-    } else if (W == 32 && L * 32 <= 8192) {
-        if constexpr (L * 32 <= 8192) k_fft_zpre<L, 32><<<grid, CGS_THREADS, 0, s>>>(a);
+    } else if (W == 16 && L * 17 <= 8192) {
+        if constexpr (L * 17 <= 8192) k_fft_zpre<L, 16><<<grid, CGS_THREADS, 0, s>>>(a);
+    } else if (W == 32 && L * 33 <= 8192) {
+        if constexpr (L * 33 <= 8192) k_fft_zpre<L, 32><<<grid, CGS_THREADS, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;
     }
@@ -507,10 +514,10 @@ static hipError_t zpost_w(int W, const ZArgs& a, long long nouter, hipStream_t s
     const dim3 grid((unsigned)nouter);
     if (W == 8) {
         k_fft_zpost<L, 8><<<grid, CGS_THREADS, 0, s>>>(a);
-    } else if (W == 16 && L * 16 <= 8192) {
-        if constexpr (L * 16 <= 8192) k_fft_zpost<L, 16><<<grid, CGS_THREADS, 0, s>>>(a);
-    } else if (W == 32 && L * 32 <= 8192) {
-        if constexpr (L * 32 <= 8192) k_fft_zpost<L, 32><<<grid, CGS_THREADS, 0, s>>>(a);
+    } else if (W == 16 && L * 17 <= 8192) {
+        if constexpr (L * 17 <= 8192) k_fft_zpost<L, 16><<<grid, CGS_THREADS, 0, s>>>(a);
+    } else if (W == 32 && L * 33 <= 8192) {
+        if constexpr (L * 33 <= 8192) k_fft_zpost<L, 32><<<grid, CGS_THREADS, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;
     }

@@ -31,6 +31,10 @@ static inline int pow2_at_least(int c, int cap) {
 }
 
 // ------------------------------------------------------------ degridding
+// The loads of one x-plane of the window (KW*KW cells) are issued without
+// conditions -- indices are clamped into the grid, and a tap outside the
+// support or below the threshold gets weight 0 -- so each lane keeps KW*KW
+// independent loads in flight instead of waiting on one tap at a time.
 template <int KW, int CW>
 __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
     constexpr int VS = val_stride(KW);
@@ -42,30 +46,42 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
     if (a.done && a.done[bin]) return;
     const int4 bs = a.s.base[s];
     const double* __restrict__ v = a.s.vals + s * VS;
+    double vx[KW], vy[KW], vz[KW];
+    int iy[KW], iz[KW];
+#pragma unroll
+    for (int o = 0; o < KW; ++o) {
+        vx[o] = v[o];
+        vy[o] = v[KW + o];
+        vz[o] = v[2 * KW + o];
+        iy[o] = min(max(bs.y + o, 0), a.g.ny - 1);
+        iz[o] = min(max(bs.z + o, 0), a.g.nz - 1);
+    }
     const int C = a.g.C;
     const float2* __restrict__ G = a.G + (long long)(bin - a.b0) * a.g.ng * C;
     const float d = a.s.dcf ? a.s.dcf[s] : 1.0f;
+    const double thr = a.thr;
     for (int c = lane_c; c < C; c += CW) {
         float2 acc = make_float2(0.0f, 0.0f);
-#pragma unroll
         for (int ox = 0; ox < KW; ++ox) {
-            const double vx = v[ox];
-            if (vx == 0.0) continue;
-            const long long xo = (long long)(bs.x + ox) * a.g.ny;
+            const double wx = vx[ox];
+            if (wx == 0.0) continue;
+            const long long xo = (long long)min(max(bs.x + ox, 0), a.g.nx - 1) * a.g.ny;
+            float2 gv[KW][KW];
 #pragma unroll
             for (int oy = 0; oy < KW; ++oy) {
-                const double vy = v[KW + oy];
-                if (vy == 0.0) continue;
-                const double vxy = vx * vy;
-                const long long row = (xo + bs.y + oy) * a.g.nz + bs.z;
+                const long long row = (xo + iy[oy]) * a.g.nz;
+#pragma unroll
+                for (int oz = 0; oz < KW; ++oz) gv[oy][oz] = G[(row + iz[oz]) * C + c];
+            }
+#pragma unroll
+            for (int oy = 0; oy < KW; ++oy) {
+                const double vxy = wx * vy[oy];
 #pragma unroll
                 for (int oz = 0; oz < KW; ++oz) {
-                    const float w = (float)(vxy * v[2 * KW + oz]);
-                    if ((double)w > a.thr) {
-                        const float2 gv = G[(row + oz) * C + c];
-                        acc.x = fmaf(w, gv.x, acc.x);
-                        acc.y = fmaf(w, gv.y, acc.y);
-                    }
+                    float w = (float)(vxy * vz[oz]);
+                    w = (double)w > thr ? w : 0.0f;
+                    acc.x = fmaf(w, gv[oy][oz].x, acc.x);
+                    acc.y = fmaf(w, gv[oy][oz].y, acc.y);
                 }
             }
         }
@@ -81,8 +97,12 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     constexpr int NPAIR = NCOL * CP;
     constexpr int NPT = (NPAIR + CGS_THREADS - 1) / CGS_THREADS;
     constexpr int NWAVE = CGS_THREADS / 64;
+    constexpr int SUB = 64;                  // candidates staged in LDS at a time
     __shared__ int list[CGS_THREADS];
     __shared__ int wcount[NWAVE];
+    __shared__ int4 sbase[SUB];
+    __shared__ double svals[SUB * VS];
+    __shared__ float2 sy[SUB * CP];
 
     const int ntiles = a.t.ntiles;
     const int lbin = blockIdx.x / ntiles;
@@ -117,7 +137,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
             const int rbeg = off[tcol + tz_lo];
             const int rend = off[tcol + tz_hi + 1];
             for (int start = rbeg; start < rend; start += CGS_THREADS) {
-                // --- compact the candidates whose window meets this tile
+                // --- compact the candidates whose window meets this tile (fixed order)
                 const int s = start + tid;
                 bool hit = false;
                 if (s < rend) {
@@ -137,31 +157,49 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
                 }
                 if (hit) list[wbase + wpre] = s;
                 __syncthreads();
-                // --- accumulate
-                for (int j = 0; j < total; ++j) {
-                    const int sj = list[j];
-                    const int4 bs = a.s.base[sj];
-                    const double* __restrict__ v = a.s.vals + (long long)sj * VS;
+                for (int j0 = 0; j0 < total; j0 += SUB) {
+                    const int nsub = min(SUB, total - j0);
+                    // --- stage window origins, KB factors and (weighted) values in LDS
+                    for (int t = tid; t < nsub * VS; t += CGS_THREADS) {
+                        const int j = t / VS, o = t % VS;
+                        svals[t] = a.s.vals[(long long)list[j0 + j] * VS + o];
+                    }
+                    for (int t = tid; t < nsub * CP; t += CGS_THREADS) {
+                        const int j = t / CP, c = t % CP;
+                        const int sj = list[j0 + j];
+                        float2 yv = make_float2(0.0f, 0.0f);
+                        if (c < C) {
+                            yv = a.Y[(long long)sj * C + c];
+                            if (a.gdcf) yv = cscale(yv, a.gdcf[sj]);
+                        }
+                        sy[t] = yv;
+                    }
+                    if (tid < nsub) sbase[tid] = a.s.base[list[j0 + tid]];
+                    __syncthreads();
+                    // --- accumulate
+                    for (int j = 0; j < nsub; ++j) {
+                        const int4 bs = sbase[j];
+                        const double* v = svals + j * VS;
 #pragma unroll
-                    for (int k = 0; k < NPT; ++k) {
-                        const int e = tid + k * CGS_THREADS;
-                        const int c = e % CP, col = e / CP;
-                        if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
-                            const int ox = x0 + col / TY - bs.x;
-                            const int oy = y0 + col % TY - bs.y;
-                            if (ox >= 0 && ox < KW && oy >= 0 && oy < KW) {
-                                const double vxy = v[ox] * v[KW + oy];
-                                if (vxy != 0.0) {
-                                    float2 yv = a.Y[(long long)sj * C + c];
-                                    if (a.gdcf) yv = cscale(yv, a.gdcf[sj]);
+                        for (int k = 0; k < NPT; ++k) {
+                            const int e = tid + k * CGS_THREADS;
+                            const int c = e % CP, col = e / CP;
+                            if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
+                                const int ox = x0 + col / TY - bs.x;
+                                const int oy = y0 + col % TY - bs.y;
+                                if (ox >= 0 && ox < KW && oy >= 0 && oy < KW) {
+                                    const double vxy = v[ox] * v[KW + oy];
+                                    if (vxy != 0.0) {
+                                        const float2 yv = sy[j * CP + c];
 #pragma unroll
-                                    for (int z = 0; z < TZ; ++z) {
-                                        const int oz = z0 + z - bs.z;
-                                        if (oz >= 0 && oz < KW) {
-                                            const float w = (float)(vxy * v[2 * KW + oz]);
-                                            if ((double)w > a.thr) {
-                                                acc[k][z].x = fmaf(w, yv.x, acc[k][z].x);
-                                                acc[k][z].y = fmaf(w, yv.y, acc[k][z].y);
+                                        for (int z = 0; z < TZ; ++z) {
+                                            const int oz = z0 + z - bs.z;
+                                            if (oz >= 0 && oz < KW) {
+                                                const float w = (float)(vxy * v[2 * KW + oz]);
+                                                if ((double)w > a.thr) {
+                                                    acc[k][z].x = fmaf(w, yv.x, acc[k][z].x);
+                                                    acc[k][z].y = fmaf(w, yv.y, acc[k][z].y);
+                                                }
                                             }
                                         }
                                     }
@@ -169,8 +207,8 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
                             }
                         }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
         }
     }

@@ -22,7 +22,7 @@ namespace cgs {
 namespace vor {
 
 constexpr int FMAX = 48;
-constexpr int VMAX = 20;
+constexpr int VMAX = 32;
 constexpr int RMAX = 10;
 
 struct V3 {
@@ -163,6 +163,12 @@ __device__ void clip(Cell& c, const V3& n, int nid) {
     c.rmax2 = r2;
 }
 
+__device__ bool has_box_face(const Cell& c) {
+    for (int f = 0; f < c.nf; f++)
+        if (c.fid[f] < 0) return true;
+    return false;
+}
+
 __device__ double cell_volume(const Cell& c) {
     double vol = 0.0;
     for (int f = 0; f < c.nf; f++) {
@@ -208,7 +214,16 @@ __global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, lo
                         const double d2 = dot3(n, n);
                         if (d2 == 0.0) continue;
                         if (0.25 * d2 >= c.rmax2) continue;
+                        const bool box = has_box_face(c);
                         clip(c, n, j);
+                        if (c.overflow) {
+                            // a cell still reaching the bounding cube that outgrows the limits
+                            // is taken as unbounded (0); a closed cell's overflow stays an error
+                            if (box) c.overflow = 0;
+                            vol[i] = 0.0;
+                            if (c.overflow) atomicAdd(overflow, 1);
+                            return;
+                        }
                     }
                 }
         const double reach = (double)r * H.h;

@@ -26,7 +26,7 @@
 namespace vor {
 
 constexpr int FMAX = 48;
-constexpr int VMAX = 20;
+constexpr int VMAX = 32;
 constexpr int RMAX = 10;   // hash shells visited at most
 
 struct V3 {
@@ -169,6 +169,12 @@ void clip(Cell& c, const V3& n, int nid) {
     c.rmax2 = r2;
 }
 
+bool has_box_face(const Cell& c) {
+    for (int f = 0; f < c.nf; f++)
+        if (c.fid[f] < 0) return true;
+    return false;
+}
+
 double cell_volume(const Cell& c) {
     double vol = 0.0;
     for (int f = 0; f < c.nf; f++) {
@@ -248,8 +254,15 @@ double point_volume(const double* p, long i, const Hash& H, double r0, Cell& c) 
                         const V3 n = sub3(q, pi);
                         const double d2 = dot3(n, n);
                         if (d2 == 0.0) continue;                 // duplicate point: same vertex
-                        if (0.25 * d2 >= c.rmax2) continue;      // bisector beyond every vertex
+                        if (0.25 * d2 >= c.rmax2) continue;
+                        const bool box = has_box_face(c);
                         clip(c, n, j);
+                        if (c.overflow) {
+                            // a cell still reaching the bounding cube that outgrows the limits
+                            // is taken as unbounded (0); a closed cell's overflow stays an error
+                            if (box) c.overflow = 0;
+                            return 0.0;
+                        }
                     }
                 }
         const double reach = (double)r * H.h;
