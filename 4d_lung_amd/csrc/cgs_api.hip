@@ -1224,6 +1224,7 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     VorHash H{};
     H.h = std::cbrt(vol * 6.0 / (double)np);
     for (int a = 0; a < 3; a++) { H.lo[a] = lo[a]; H.n[a] = (int)((hi[a] - lo[a]) / H.h) + 1; }
+    for (int a = 0; a < 3; a++) { H.blo[a] = lo[a] - H.h; H.bhi[a] = hi[a] + H.h; }
     const long long nc = (long long)H.n[0] * H.n[1] * H.n[2];
     std::vector<int> start(nc + 1, 0), key(np), idx(np);
     for (long long i = 0; i < np; i++) {
@@ -1238,9 +1239,6 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     for (long long c = 0; c < nc; c++) start[c + 1] += start[c];
     std::vector<int> cur(start.begin(), start.end() - 1);
     for (long long i = 0; i < np; i++) idx[cur[key[i]]++] = (int)i;
-    double ext = 0.0;
-    for (int a = 0; a < 3; a++) ext = std::fmax(ext, H.h * H.n[a]);
-    const double r0 = 4.0 * ext;
 
     double *dp = nullptr, *dv = nullptr;
     int *ds = nullptr, *di = nullptr, *dov = nullptr;
@@ -1255,7 +1253,7 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     CK(hipMemset(dov, 0, sizeof(int)));
     H.start = ds;
     H.idx = di;
-    CK(launch_voronoi(dp, np, H, r0, dv, dov, st));
+    CK(launch_voronoi(dp, np, H, dv, dov, st));
     int over = 0;
     CK(hipMemcpyAsync(vol_host, dv, sizeof(double) * np, hipMemcpyDeviceToHost, st));
     CK(hipMemcpyAsync(&over, dov, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -197,17 +197,17 @@ struct Dft<7, DIR> : DftOdd<7, DIR> {};
 // Stage with radix R on sub-transforms of size LS (Stockham autosort):
 //   butterfly j reads l = j + i*L/R, twiddles by exp(-+2 pi i k i/(LS R)) with
 //   k = j mod LS, and writes l = (j/LS)*LS*R + k + i*LS.
-template <int L, int R, int LS, int DIR, int NCOL, int STRIDE>
+template <int L, int R, int LS, int DIR, int NCOL, int STRIDE, int NT = CGS_THREADS>
 __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict__ tw) {
     constexpr int NB = NCOL * (L / R);
-    constexpr int PER = (NB + CGS_THREADS - 1) / CGS_THREADS;
+    constexpr int PER = (NB + NT - 1) / NT;
     constexpr int TS = L / (LS * R);
     const int tid = threadIdx.x;
     float2 v[PER][R];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        const int b = tid + p * CGS_THREADS;
-        if ((NB % CGS_THREADS) == 0 || b < NB) {
+        const int b = tid + p * NT;
+        if ((NB % NT) == 0 || b < NB) {
             const int col = b % NCOL, j = b / NCOL, k = j % LS;
 #pragma unroll
             for (int i = 0; i < R; ++i) {
@@ -224,8 +224,8 @@ __device__ __forceinline__ void fft_stage(float2* lds, const float2* __restrict_
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        const int b = tid + p * CGS_THREADS;
-        if ((NB % CGS_THREADS) == 0 || b < NB) {
+        const int b = tid + p * NT;
+        if ((NB % NT) == 0 || b < NB) {
             const int col = b % NCOL, j = b / NCOL, k = j % LS;
             Dft<R, DIR>::run(v[p]);
             const int base = (j / LS) * LS * R + k;
@@ -245,34 +245,166 @@ __device__ __forceinline__ void fft_stages(float2* lds, const float2* __restrict
     }
 }
 
+// Stages S .. SEND-1 through LDS (each ends with a barrier).
+template <int L, int S, int SEND, int LS, int DIR, int NCOL, int STRIDE, int NT>
+__device__ __forceinline__ void fft_mid_stages(float2* lds, const float2* tw) {
+    if constexpr (S < SEND) {
+        constexpr int R = plan_radix(L, S);
+        fft_stage<L, R, LS, DIR, NCOL, STRIDE, NT>(lds, tw);
+        fft_mid_stages<L, S + 1, SEND, LS * R, DIR, NCOL, STRIDE, NT>(lds, tw);
+    }
+}
+
+// Product of the radices of stages 0 .. S-1 (sub-transform size before stage S).
+__host__ __device__ constexpr int plan_ls(int L, int S) {
+    int ls = 1;
+    for (int i = 0; i < S; ++i) ls *= plan_radix(L, i);
+    return ls;
+}
+
+// Whole line transform of an [L x NCOL] tile whose first stage takes its
+// inputs straight from ld(l, col) (HBM -> registers) and whose last stage
+// hands its outputs straight to st(l, col, v) (registers -> HBM or an
+// epilogue); only the stages in between go through LDS.  tw: L twiddles
+// exp(-2 pi i m / L), visible to the block after the first barrier.  With
+// LDS_STORE the last stage waits at a barrier between its LDS reads and the
+// stores, so that st may write the tile.  Ends without a barrier.
+// With FUSED_FIRST false the tile is first loaded into LDS element by element
+// (consecutive threads on consecutive columns) and every stage but the last
+// runs through LDS; this needs fewer registers when ld is expensive.
+template <int L, int DIR, int NCOL, int STRIDE, bool LDS_STORE, bool FUSED_FIRST = true, int NT = CGS_THREADS,
+          class Load, class Store>
+__device__ __forceinline__ void fft_line_tile(float2* lds, const float2* tw, Load ld, Store st) {
+    constexpr int NS = plan_nstages(L);
+    const int tid = threadIdx.x;
+    if constexpr (!FUSED_FIRST && NS > 1) {
+        for (int e = tid; e < L * NCOL; e += NT) {
+            const int c = e % NCOL, l = e / NCOL;
+            lds[l * STRIDE + c] = ld(l, c);
+        }
+        __syncthreads();
+        fft_mid_stages<L, 0, NS - 1, 1, DIR, NCOL, STRIDE, NT>(lds, tw);
+    } else if constexpr (NS == 1) {
+        constexpr int R = L;
+        constexpr int PER = (NCOL + NT - 1) / NT;
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int col = tid + p * NT;
+            if (col < NCOL) {
+                float2 v[R];
+#pragma unroll
+                for (int i = 0; i < R; ++i) v[i] = ld(i, col);
+                Dft<R, DIR>::run(v);
+#pragma unroll
+                for (int i = 0; i < R; ++i) st(i, col, v[i]);
+            }
+        }
+    } else {
+        {   // first stage: LS = 1, no twiddles
+            constexpr int R = plan_radix(L, 0);
+            constexpr int NB = NCOL * (L / R);
+            constexpr int PER = (NB + NT - 1) / NT;
+            float2 v[PER][R];
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                const int b = tid + p * NT;
+                if ((NB % NT) == 0 || b < NB) {
+                    const int col = b % NCOL, j = b / NCOL;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) v[p][i] = ld(j + i * (L / R), col);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                const int b = tid + p * NT;
+                if ((NB % NT) == 0 || b < NB) {
+                    const int col = b % NCOL, j = b / NCOL;
+                    Dft<R, DIR>::run(v[p]);
+#pragma unroll
+                    for (int i = 0; i < R; ++i) lds[(j * R + i) * STRIDE + col] = v[p][i];
+                }
+            }
+            __syncthreads();
+        }
+        fft_mid_stages<L, 1, NS - 1, plan_radix(L, 0), DIR, NCOL, STRIDE, NT>(lds, tw);
+    }
+    if constexpr (NS > 1) {
+        {   // last stage: LS * R = L, output line index k + i * LS with k = j
+            constexpr int R = plan_radix(L, NS - 1);
+            constexpr int LS = plan_ls(L, NS - 1);
+            constexpr int NB = NCOL * (L / R);
+            constexpr int PER = (NB + NT - 1) / NT;
+            float2 v[PER][R];
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                const int b = tid + p * NT;
+                if ((NB % NT) == 0 || b < NB) {
+                    const int col = b % NCOL, j = b / NCOL;
+#pragma unroll
+                    for (int i = 0; i < R; ++i) {
+                        float2 x = lds[(j + i * LS) * STRIDE + col];
+                        if (i > 0) {
+                            float2 t = tw[j * i];
+                            if (DIR > 0) t.y = -t.y;
+                            x = cmul(x, t);
+                        }
+                        v[p][i] = x;
+                    }
+                }
+            }
+            if constexpr (LDS_STORE) __syncthreads();
+#pragma unroll
+            for (int p = 0; p < PER; ++p) {
+                const int b = tid + p * NT;
+                if ((NB % NT) == 0 || b < NB) {
+                    const int col = b % NCOL, j = b / NCOL;
+                    Dft<R, DIR>::run(v[p]);
+#pragma unroll
+                    for (int i = 0; i < R; ++i) st(j + i * LS, col, v[p][i]);
+                }
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ float2 alt_sign(float2 v, int l) {
     return (l & 1) ? make_float2(-v.x, -v.y) : v;
 }
 
 // ------------------------------------------------------------ pass kernels
+// First stage of the z passes: the forward pass (three loads per element)
+// stages its tile through LDS, the inverse pass loads into registers.
+#ifndef CGS_ZPRE_FUSED_FIRST
+#define CGS_ZPRE_FUSED_FIRST false
+#endif
+#ifndef CGS_ZPOST_FUSED_FIRST
+#define CGS_ZPOST_FUSED_FIRST true
+#endif
+
 // Strided pass (y or x axis): block = one outer index x one chunk of W
 // consecutive columns; lines of length L with stride a.ls.
 template <int L, int W, int DIR>
 __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
+    // LDS holds only the tile (it bounds the blocks per CU here); twiddles
+    // are read through the L1 cache
     __shared__ float2 lds[L * W];
     const long long nchunk = (a.ncols + W - 1) / W;
     const long long bid = blockIdx.x;
     const long long chunk = bid % nchunk, outer = bid / nchunk;
     if (a.done && a.done[outer / a.outer_per_bin]) return;
     float2* g = a.g + outer * a.os + chunk * W;
-    const long long ncol_here = a.ncols - chunk * W;
-    for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-        const int c = e % W, l = e / W;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (c < ncol_here) v = g[(long long)l * a.ls + c];
-        lds[e] = alt_sign(v, l);
-    }
-    __syncthreads();
-    fft_stages<L, 0, 1, DIR, W>(lds, a.tw);
-    for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-        const int c = e % W, l = e / W;
-        if (c < ncol_here) g[(long long)l * a.ls + c] = alt_sign(lds[e], l);
-    }
+    const long long ls = a.ls;
+    const int ncol_here = (int)min((long long)W, a.ncols - chunk * W);
+    fft_line_tile<L, DIR, W, W, false>(
+        lds, a.tw,
+        [&](int l, int c) {
+            float2 v = make_float2(0.0f, 0.0f);
+            if (c < ncol_here) v = g[(long long)l * ls + c];
+            return alt_sign(v, l);
+        },
+        [&](int l, int c, float2 v) {
+            if (c < ncol_here) g[(long long)l * ls + c] = alt_sign(v, l);
+        });
 }
 
 // z pass with a single column (one complex grid, C = 1): block = OB lines.
@@ -299,35 +431,36 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_z1(float2* g, long long nli
 //   a.p != null : value = p[bin][cell] * S[cell][c]      (elementProduct, Reco.cpp:85-93)
 //   a.p == null : value = G[bin][cell][c]                 (coil image given)
 //   a.kft != null: value *= kft[cell]                     (deapodization, Reco.cpp:529-535)
-template <int L, int W>
-__global__ __launch_bounds__(CGS_THREADS) void k_fft_zpre(ZArgs a) {
+template <int L, int W, int NT>
+__global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     __shared__ float2 lds[L * W];
+    __shared__ float2 twl[L];
     const long long outer = blockIdx.x;             // bin * nxy + xy
     const long long bin = outer / a.nxy, xy = outer % a.nxy;
     if (a.done && a.done[bin]) return;
+    for (int e = threadIdx.x; e < L; e += NT) twl[e] = a.tw[e];
     const long long cell0 = xy * L;                  // cell index of (x, y, z=0)
     float2* g = a.g + (bin * a.ng + cell0) * a.C;
-    for (int c0 = 0; c0 < a.C; c0 += W) {
-        for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-            const int c = e % W, l = e / W;
-            const int coil = c0 + c;
-            float2 v = make_float2(0.0f, 0.0f);
-            if (coil < a.C) {
-                const long long cell = cell0 + l;
-                if (a.p) v = cmul(a.p[bin * a.ng + cell], a.maps[cell * a.C + coil]);
-                else v = g[(long long)l * a.C + coil];
-                if (a.kft) v = cscale(v, a.kft[cell]);
-            }
-            lds[e] = alt_sign(v, l);
-        }
-        __syncthreads();
-        fft_stages<L, 0, 1, -1, W>(lds, a.tw);
-        for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-            const int c = e % W, l = e / W;
-            const int coil = c0 + c;
-            if (coil < a.C) g[(long long)l * a.C + coil] = alt_sign(lds[e], l);
-        }
-        __syncthreads();
+    const int C = a.C;
+    for (int c0 = 0; c0 < C; c0 += W) {
+        fft_line_tile<L, -1, W, W, false, CGS_ZPRE_FUSED_FIRST, NT>(
+            lds, twl,
+            [&](int l, int c) {
+                const int coil = c0 + c;
+                float2 v = make_float2(0.0f, 0.0f);
+                if (coil < C) {
+                    const long long cell = cell0 + l;
+                    if (a.p) v = cmul(a.p[bin * a.ng + cell], a.maps[cell * C + coil]);
+                    else v = g[(long long)l * C + coil];
+                    if (a.kft) v = cscale(v, a.kft[cell]);
+                }
+                return alt_sign(v, l);
+            },
+            [&](int l, int c, float2 v) {
+                const int coil = c0 + c;
+                if (coil < C) g[(long long)l * C + coil] = alt_sign(v, l);
+            });
+        if (c0 + W < C) __syncthreads();
     }
 }
 
@@ -341,53 +474,57 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_zpre(ZArgs a) {
 //                    element) and one thread per z sums its row over the coils;
 //                    the block's partial <p, q> goes to a.part[blockIdx.x]
 //   a.maps == null : G[bin][cell][c] = value (coil images, nufftBWD)
-template <int L, int W>
-__global__ __launch_bounds__(CGS_THREADS) void k_fft_zpost(ZArgs a) {
+template <int L, int W, int NT>
+__global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
     constexpr int LDW = W + 1;
-    constexpr int RPT = (L + CGS_THREADS - 1) / CGS_THREADS;
+    constexpr int RPT = (L + NT - 1) / NT;
     __shared__ float2 lds[L * LDW];
-    __shared__ double red[2 * (CGS_THREADS / 64)];
+    __shared__ float2 twl[L];
+    __shared__ double red[2 * (NT / 64)];
     const long long outer = blockIdx.x;
     const long long bin = outer / a.nxy, xy = outer % a.nxy;
     if (a.done && a.done[bin]) return;
+    for (int e = threadIdx.x; e < L; e += NT) twl[e] = a.tw[e];
     const long long cell0 = xy * L;
     const int gx = (int)(xy / a.ny), gy = (int)(xy % a.ny);
     const bool xy_in = !a.crop || (gx >= a.lo[0] && gx <= a.hi[0] && gy >= a.lo[1] && gy <= a.hi[1]);
     float2* g = a.g + (bin * a.ng + cell0) * a.C;
     const bool combine = a.maps != nullptr;
+    const int C = a.C;
     float2 acc[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) acc[r] = make_float2(0.0f, 0.0f);
-    for (int c0 = 0; c0 < a.C; c0 += W) {
-        for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-            const int c = e % W, l = e / W;
-            const int coil = c0 + c;
-            float2 v = make_float2(0.0f, 0.0f);
-            if (coil < a.C) v = g[(long long)l * a.C + coil];
-            lds[l * LDW + c] = alt_sign(v, l);
-        }
-        __syncthreads();
-        fft_stages<L, 0, 1, +1, W, LDW>(lds, a.tw);
-        for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-            const int c = e % W, l = e / W;
-            const int coil = c0 + c;
-            const long long cell = cell0 + l;
-            float2 v = make_float2(0.0f, 0.0f);
-            if (coil < a.C) {
-                v = alt_sign(lds[l * LDW + c], l);
-                const float sc = a.kft ? a.kft[cell] * a.scale : a.scale;
-                v = cscale(v, sc);
-                if (!xy_in || (a.crop && (l < a.lo[2] || l > a.hi[2]))) v = make_float2(0.0f, 0.0f);
-                if (combine) v = cmulc(a.maps[cell * a.C + coil], v);
-                else g[(long long)l * a.C + coil] = v;
-            }
-            if (combine) lds[l * LDW + c] = v;
-        }
+    for (int c0 = 0; c0 < C; c0 += W) {
+        // inverse transform; the epilogue runs on the last stage's outputs:
+        // products with conj(S) go back into the tile for the coil sum, coil
+        // images go straight to the grid
+        fft_line_tile<L, +1, W, LDW, true, CGS_ZPOST_FUSED_FIRST, NT>(
+            lds, twl,
+            [&](int l, int c) {
+                const int coil = c0 + c;
+                float2 v = make_float2(0.0f, 0.0f);
+                if (coil < C) v = g[(long long)l * C + coil];
+                return alt_sign(v, l);
+            },
+            [&](int l, int c, float2 v) {
+                const int coil = c0 + c;
+                const long long cell = cell0 + l;
+                float2 r = make_float2(0.0f, 0.0f);
+                if (coil < C) {
+                    r = alt_sign(v, l);
+                    const float sc = a.kft ? a.kft[cell] * a.scale : a.scale;
+                    r = cscale(r, sc);
+                    if (!xy_in || (a.crop && (l < a.lo[2] || l > a.hi[2]))) r = make_float2(0.0f, 0.0f);
+                    if (combine) r = cmulc(a.maps[cell * C + coil], r);
+                    else g[(long long)l * C + coil] = r;
+                }
+                if (combine) lds[l * LDW + c] = r;
+            });
         if (combine) {
             __syncthreads();
 #pragma unroll
             for (int r = 0; r < RPT; ++r) {
-                const int l = threadIdx.x + r * CGS_THREADS;
+                const int l = threadIdx.x + r * NT;
                 if (l < L) {
                     float2 sum = make_float2(0.0f, 0.0f);
 #pragma unroll
@@ -396,13 +533,13 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_zpost(ZArgs a) {
                 }
             }
         }
-        __syncthreads();
+        if (c0 + W < C) __syncthreads();
     }
     if (!combine) return;
     double pr = 0.0, pi = 0.0;
 #pragma unroll
     for (int r = 0; r < RPT; ++r) {
-        const int l = threadIdx.x + r * CGS_THREADS;
+        const int l = threadIdx.x + r * NT;
         if (l < L) {
             const long long idx = bin * a.ng + cell0 + l;
             a.q[idx] = acc[r];
@@ -423,7 +560,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_zpost(ZArgs a) {
         __syncthreads();
         if (threadIdx.x == 0) {
             double sr = 0.0, si = 0.0;
-            for (int w = 0; w < CGS_THREADS / 64; ++w) { sr += red[2 * w]; si += red[2 * w + 1]; }
+            for (int w = 0; w < NT / 64; ++w) { sr += red[2 * w]; si += red[2 * w + 1]; }
             a.part[2 * blockIdx.x] = sr;
             a.part[2 * blockIdx.x + 1] = si;
         }
@@ -447,12 +584,17 @@ bool fft_len_supported(int L) {
 // Column chunk of the strided passes: keep the LDS tile at or below 32K complex.
 static inline int plain_w(int L) { return L <= 256 ? 32 : 16; }
 
-// Coil chunk of the z pass: the whole coil set when the LDS tile stays at or
-// below 8K complex (64 KB), else the largest power of two that fits.
+// Coil chunk of the z pass: 8 coils per LDS tile (small tiles keep four or
+// more blocks resident per CU, which hides the HBM latency better than wider
+// tiles with fewer blocks).
 int zpass_w(int L, int C) {
-    int w = C <= 8 ? 8 : (C <= 16 ? 16 : 32);
-    while (w > 8 && L * (w + 1) > 8192) w /= 2;
-    return w;
+    (void)L;
+    (void)C;
+#ifdef CGS_ZW
+    return CGS_ZW;
+#else
+    return 8;
+#endif
 }
 
 template <int L, int DIR>
@@ -494,15 +636,16 @@ hipError_t launch_fft_z1(int L, int dir, float2* g, long long nlines, const floa
     return hipGetLastError();
 }
 
+// z-pass block shape: W coils per LDS tile, NT threads (16-coil tiles use
+// 512 threads so that each thread keeps the register footprint of the
+// 8-coil, 256-thread shape).
 template <int L>
 static hipError_t zpre_w(int W, const ZArgs& a, long long nouter, hipStream_t s) {
     const dim3 grid((unsigned)nouter);
     if (W == 8) {
-        k_fft_zpre<L, 8><<<grid, CGS_THREADS, 0, s>>>(a);
+        k_fft_zpre<L, 8, 256><<<grid, 256, 0, s>>>(a);
     } else if (W == 16 && L * 17 <= 8192) {
-        if constexpr (L * 17 <= 8192) k_fft_zpre<L, 16><<<grid, CGS_THREADS, 0, s>>>(a);
-    } else if (W == 32 && L * 33 <= 8192) {
-        if constexpr (L * 33 <= 8192) k_fft_zpre<L, 32><<<grid, CGS_THREADS, 0, s>>>(a);
+        if constexpr (L * 17 <= 8192) k_fft_zpre<L, 16, 512><<<grid, 512, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;
     }
@@ -513,11 +656,9 @@ template <int L>
 static hipError_t zpost_w(int W, const ZArgs& a, long long nouter, hipStream_t s) {
     const dim3 grid((unsigned)nouter);
     if (W == 8) {
-        k_fft_zpost<L, 8><<<grid, CGS_THREADS, 0, s>>>(a);
+        k_fft_zpost<L, 8, 256><<<grid, 256, 0, s>>>(a);
     } else if (W == 16 && L * 17 <= 8192) {
-        if constexpr (L * 17 <= 8192) k_fft_zpost<L, 16><<<grid, CGS_THREADS, 0, s>>>(a);
-    } else if (W == 32 && L * 33 <= 8192) {
-        if constexpr (L * 33 <= 8192) k_fft_zpost<L, 32><<<grid, CGS_THREADS, 0, s>>>(a);
+        if constexpr (L * 17 <= 8192) k_fft_zpost<L, 16, 512><<<grid, 512, 0, s>>>(a);
     } else {
         return hipErrorInvalidValue;
     }

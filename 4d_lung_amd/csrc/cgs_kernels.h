@@ -143,13 +143,14 @@ hipError_t launch_scale_d(float2* a, long long n, double f, hipStream_t s);
 // in ascending point order, c = (x*n1 + y)*n2 + z.
 struct VorHash {
     double lo[3];
+    double blo[3], bhi[3];   // bounding box of the points widened by one hash cell
     double h;
     int n[3];
     const int* start;
     const int* idx;
 };
 
-hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double r0, double* vol, int* overflow,
+hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* overflow,
                           hipStream_t s);
 
 }  // namespace cgs

@@ -100,6 +100,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     constexpr int SUB = 64;                  // candidates staged in LDS at a time
     __shared__ int list[CGS_THREADS];
     __shared__ int wcount[NWAVE];
+    __shared__ int rbeg[9], rlen[9];
     __shared__ int4 sbase[SUB];
     __shared__ double svals[SUB * VS];
     __shared__ float2 sy[SUB * CP];
@@ -127,89 +128,103 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     const int tz_lo = itz > 0 ? itz - 1 : 0;
     const int tz_hi = itz + 1 < a.t.ntz ? itz + 1 : a.t.ntz - 1;
 
-    for (int dx = -1; dx <= 1; ++dx) {
-        const int jx = itx + dx;
-        if (jx < 0 || jx >= a.t.ntx) continue;
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int jy = ity + dy;
-            if (jy < 0 || jy >= a.t.nty) continue;
+    // The candidates are the samples of the 3 x 3 tile columns around this one
+    // (z tiles itz-1 .. itz+1): nine CSR ranges, walked as one flat sequence
+    // in (dx, dy, sample) order.
+    if (tid < 9) {
+        const int jx = itx + tid / 3 - 1, jy = ity + tid % 3 - 1;
+        int rb = 0, rl = 0;
+        if (jx >= 0 && jx < a.t.ntx && jy >= 0 && jy < a.t.nty) {
             const int tcol = (jx * a.t.nty + jy) * a.t.ntz;
-            const int rbeg = off[tcol + tz_lo];
-            const int rend = off[tcol + tz_hi + 1];
-            for (int start = rbeg; start < rend; start += CGS_THREADS) {
-                // --- compact the candidates whose window meets this tile (fixed order)
-                const int s = start + tid;
-                bool hit = false;
-                if (s < rend) {
-                    const int4 bs = a.s.base[s];
-                    hit = bs.x < x0 + TX && bs.x + KW > x0 && bs.y < y0 + TY && bs.y + KW > y0 &&
-                          bs.z < z0 + TZ && bs.z + KW > z0;
+            rb = off[tcol + tz_lo];
+            rl = off[tcol + tz_hi + 1] - rb;
+        }
+        rbeg[tid] = rb;
+        rlen[tid] = rl;
+    }
+    __syncthreads();
+    int rpre[10];
+    rpre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) rpre[r + 1] = rpre[r] + rlen[r];
+    const int nflat = rpre[9];
+    for (int f0 = 0; f0 < nflat; f0 += CGS_THREADS) {
+        // --- compact the candidates whose window meets this tile (fixed order)
+        const int f = f0 + tid;
+        bool hit = false;
+        int s = 0;
+        if (f < nflat) {
+            int r = 0;
+#pragma unroll
+            for (int q = 1; q < 9; ++q) r += f >= rpre[q] ? 1 : 0;
+            s = rbeg[r] + (f - rpre[r]);
+            const int4 bs = a.s.base[s];
+            hit = bs.x < x0 + TX && bs.x + KW > x0 && bs.y < y0 + TY && bs.y + KW > y0 &&
+                  bs.z < z0 + TZ && bs.z + KW > z0;
+        }
+        const unsigned long long bal = __ballot(hit);
+        const int wpre = __popcll(bal & ((1ULL << lane) - 1ULL));
+        if (lane == 0) wcount[wid] = __popcll(bal);
+        __syncthreads();
+        int wbase = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) {
+            if (w < wid) wbase += wcount[w];
+            total += wcount[w];
+        }
+        if (hit) list[wbase + wpre] = s;
+        __syncthreads();
+        for (int j0 = 0; j0 < total; j0 += SUB) {
+            const int nsub = min(SUB, total - j0);
+            // --- stage window origins, KB factors and (weighted) values in LDS
+            for (int t = tid; t < nsub * VS; t += CGS_THREADS) {
+                const int j = t / VS, o = t % VS;
+                svals[t] = a.s.vals[(long long)list[j0 + j] * VS + o];
+            }
+            for (int t = tid; t < nsub * CP; t += CGS_THREADS) {
+                const int j = t / CP, c = t % CP;
+                const int sj = list[j0 + j];
+                float2 yv = make_float2(0.0f, 0.0f);
+                if (c < C) {
+                    yv = a.Y[(long long)sj * C + c];
+                    if (a.gdcf) yv = cscale(yv, a.gdcf[sj]);
                 }
-                const unsigned long long bal = __ballot(hit);
-                const int wpre = __popcll(bal & ((1ULL << lane) - 1ULL));
-                if (lane == 0) wcount[wid] = __popcll(bal);
-                __syncthreads();
-                int wbase = 0, total = 0;
+                sy[t] = yv;
+            }
+            if (tid < nsub) sbase[tid] = a.s.base[list[j0 + tid]];
+            __syncthreads();
+            // --- accumulate
+            for (int j = 0; j < nsub; ++j) {
+                const int4 bs = sbase[j];
+                const double* v = svals + j * VS;
 #pragma unroll
-                for (int w = 0; w < NWAVE; ++w) {
-                    if (w < wid) wbase += wcount[w];
-                    total += wcount[w];
-                }
-                if (hit) list[wbase + wpre] = s;
-                __syncthreads();
-                for (int j0 = 0; j0 < total; j0 += SUB) {
-                    const int nsub = min(SUB, total - j0);
-                    // --- stage window origins, KB factors and (weighted) values in LDS
-                    for (int t = tid; t < nsub * VS; t += CGS_THREADS) {
-                        const int j = t / VS, o = t % VS;
-                        svals[t] = a.s.vals[(long long)list[j0 + j] * VS + o];
-                    }
-                    for (int t = tid; t < nsub * CP; t += CGS_THREADS) {
-                        const int j = t / CP, c = t % CP;
-                        const int sj = list[j0 + j];
-                        float2 yv = make_float2(0.0f, 0.0f);
-                        if (c < C) {
-                            yv = a.Y[(long long)sj * C + c];
-                            if (a.gdcf) yv = cscale(yv, a.gdcf[sj]);
-                        }
-                        sy[t] = yv;
-                    }
-                    if (tid < nsub) sbase[tid] = a.s.base[list[j0 + tid]];
-                    __syncthreads();
-                    // --- accumulate
-                    for (int j = 0; j < nsub; ++j) {
-                        const int4 bs = sbase[j];
-                        const double* v = svals + j * VS;
+                for (int k = 0; k < NPT; ++k) {
+                    const int e = tid + k * CGS_THREADS;
+                    const int c = e % CP, col = e / CP;
+                    if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
+                        const int ox = x0 + col / TY - bs.x;
+                        const int oy = y0 + col % TY - bs.y;
+                        if (ox >= 0 && ox < KW && oy >= 0 && oy < KW) {
+                            const double vxy = v[ox] * v[KW + oy];
+                            if (vxy != 0.0) {
+                                const float2 yv = sy[j * CP + c];
 #pragma unroll
-                        for (int k = 0; k < NPT; ++k) {
-                            const int e = tid + k * CGS_THREADS;
-                            const int c = e % CP, col = e / CP;
-                            if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
-                                const int ox = x0 + col / TY - bs.x;
-                                const int oy = y0 + col % TY - bs.y;
-                                if (ox >= 0 && ox < KW && oy >= 0 && oy < KW) {
-                                    const double vxy = v[ox] * v[KW + oy];
-                                    if (vxy != 0.0) {
-                                        const float2 yv = sy[j * CP + c];
-#pragma unroll
-                                        for (int z = 0; z < TZ; ++z) {
-                                            const int oz = z0 + z - bs.z;
-                                            if (oz >= 0 && oz < KW) {
-                                                const float w = (float)(vxy * v[2 * KW + oz]);
-                                                if ((double)w > a.thr) {
-                                                    acc[k][z].x = fmaf(w, yv.x, acc[k][z].x);
-                                                    acc[k][z].y = fmaf(w, yv.y, acc[k][z].y);
-                                                }
-                                            }
+                                for (int z = 0; z < TZ; ++z) {
+                                    const int oz = z0 + z - bs.z;
+                                    if (oz >= 0 && oz < KW) {
+                                        const float w = (float)(vxy * v[2 * KW + oz]);
+                                        if ((double)w > a.thr) {
+                                            acc[k][z].x = fmaf(w, yv.x, acc[k][z].x);
+                                            acc[k][z].y = fmaf(w, yv.y, acc[k][z].y);
                                         }
                                     }
                                 }
                             }
                         }
                     }
-                    __syncthreads();
                 }
             }
+            __syncthreads();
         }
     }
     // --- write the tile (every cell of the grid belongs to exactly one tile)

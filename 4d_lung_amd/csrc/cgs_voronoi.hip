@@ -4,13 +4,16 @@
 // The reference takes each sample's Voronoi cell volume from a CGAL Delaunay
 // triangulation (sum over Delaunay edges of dual-facet area * edge length / 6,
 // 0 for points on the convex hull) and divides by the maximum.  Here every
-// cell is built on its own: start from a cube around the point and intersect
-// it with the bisector half-spaces of the other points, visiting a uniform
-// hash grid shell by shell in a fixed order until no unvisited point can cut
-// the cell (the farthest vertex is closer than half the distance of the next
-// shell).  Cells that keep a face of the cube after the last shell are taken
-// as unbounded (volume 0).  The volume is the reference's face sum
-// area * |q - p| / 6 with areas as triangle fans from the first vertex.
+// cell is built on its own: start from the bounding box of all points widened
+// by one hash cell and intersect it with the bisector half-spaces of the other
+// points, visiting a uniform hash grid shell by shell in a fixed order until
+// no unvisited point can cut the cell (the farthest vertex is closer than half
+// the distance of the next shell).  Cells that keep a face of the box are
+// taken as unbounded (volume 0): the hull points, and interior points whose
+// Voronoi region reaches more than one hash cell beyond the point cloud.  The
+// volume is the reference's face sum area * |q - p| / 6 with areas as
+// triangle fans from the first vertex.  A cell is a pool of vertices and
+// faces as rings of pool indices (about 8.5 KB per thread).
 //
 // Built with -ffp-contract=off; every operation is a separately rounded
 // IEEE double operation in a fixed order, so the weights are reproducible
@@ -21,22 +24,28 @@
 namespace cgs {
 namespace vor {
 
-constexpr int FMAX = 48;
-constexpr int VMAX = 32;
-constexpr int RMAX = 10;
+constexpr int FMAX = 48;   // faces of a cell
+constexpr int VMAX = 32;   // vertices of one face
+constexpr int PMAX = 96;   // distinct vertices of a cell (a simple polyhedron with F faces has <= 2F - 4)
+constexpr int RMAX = 10;   // hash shells visited at most
 
 struct V3 {
     double x, y, z;
 };
 
+// Faces are rings of indices into a vertex pool; clipping writes the new
+// pool into the other half of v[2][] and flips `cur`.
 struct Cell {
     int nf;
+    int nvert;
+    int cur;
+    int overflow;
+    double rmax2;
     int fid[FMAX];
     double fh[FMAX];
-    int nv[FMAX];
-    V3 v[FMAX][VMAX];
-    double rmax2;
-    int overflow;
+    unsigned char nv[FMAX];
+    unsigned char ring[FMAX][VMAX];
+    V3 v[2][PMAX];
 };
 
 __device__ __forceinline__ double dot3(const V3& a, const V3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
@@ -45,22 +54,32 @@ __device__ __forceinline__ V3 cross3(const V3& a, const V3& b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-__device__ void init_box(Cell& c, double r0) {
-    const signed char q[6][4][3] = {
-        {{1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {1, -1, 1}},     {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, 1}, {-1, 1, -1}},
-        {{-1, 1, -1}, {-1, 1, 1}, {1, 1, 1}, {1, 1, -1}},     {{-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}},
-        {{-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}},     {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1}}};
+__device__ void init_box(Cell& c, const double* lo, const double* hi) {
+    // the bounding box [lo, hi] of all points (plus margin), relative to the
+    // cell's point; corner k has coordinates (bit 2, bit 1, bit 0) of k
+    const unsigned char q[6][4] = {{4, 6, 7, 5}, {0, 1, 3, 2}, {2, 3, 7, 6},
+                                   {0, 4, 5, 1}, {1, 5, 7, 3}, {0, 2, 6, 4}};
     c.nf = 6;
+    c.nvert = 8;
+    c.cur = 0;
     c.overflow = 0;
+    for (int k = 0; k < 8; k++)
+        c.v[0][k] = {(k & 4) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1], (k & 1) ? hi[2] : lo[2]};
+    double r2 = 0.0;
     for (int f = 0; f < 6; f++) {
         c.fid[f] = -1;
         c.fh[f] = 0.0;
         c.nv[f] = 4;
-        for (int k = 0; k < 4; k++) c.v[f][k] = {q[f][k][0] * r0, q[f][k][1] * r0, q[f][k][2] * r0};
+        for (int k = 0; k < 4; k++) {
+            c.ring[f][k] = q[f][k];
+            const double d = dot3(c.v[0][q[f][k]], c.v[0][q[f][k]]);
+            if (d > r2) r2 = d;
+        }
     }
-    c.rmax2 = 3.0 * r0 * r0;
+    c.rmax2 = r2;
 }
 
+// Angular order of cap points around their centroid in the plane with normal n.
 __device__ __forceinline__ bool ang_less(double a1, double b1, double a2, double b2) {
     const bool u1 = b1 > 0.0 || (b1 == 0.0 && a1 > 0.0);
     const bool u2 = b2 > 0.0 || (b2 == 0.0 && a2 > 0.0);
@@ -68,73 +87,114 @@ __device__ __forceinline__ bool ang_less(double a1, double b1, double a2, double
     return a1 * b2 - a2 * b1 > 0.0;
 }
 
+__device__ __forceinline__ bool same3(const V3& a, const V3& b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// Intersect the cell with the half-space n.x <= |n|^2/2.  Vertices with
+// signed distance within +-tol of the plane are kept and belong to the cap;
+// an edge whose ends lie strictly on both sides gets one new vertex,
+// interpolated from its inner end, shared by the two faces of the edge.  The
+// cap (the new face) is the set of distinct cap vertices, in first-seen order
+// over the faces, sorted by angle around their mean.
 __device__ void clip(Cell& c, const V3& n, int nid) {
     const double cc = 0.5 * dot3(n, n);
     const double tol = 1e-10 * cc;
+    const V3* __restrict__ ov = c.v[c.cur];
+    V3* __restrict__ nvx = c.v[c.cur ^ 1];
+    double s[PMAX];
+    for (int i = 0; i < c.nvert; i++) s[i] = dot3(n, ov[i]) - cc;
     bool any_out = false;
     for (int f = 0; f < c.nf && !any_out; f++)
         for (int k = 0; k < c.nv[f]; k++)
-            if (dot3(n, c.v[f][k]) - cc > tol) { any_out = true; break; }
+            if (s[c.ring[f][k]] > tol) { any_out = true; break; }
     if (!any_out) return;
-    V3 cap[FMAX * 2];
+    // new-pool index of every kept old vertex (255 = not yet placed), and for
+    // each new vertex the old edge it was cut from (0xffff for kept vertices)
+    unsigned char map[PMAX];
+    unsigned short edge[PMAX];
+    for (int i = 0; i < c.nvert; i++) map[i] = 255;
+    int nn = 0;
+    unsigned char cap[FMAX * 2];
     int ncap = 0;
     int nf = 0;
     for (int f = 0; f < c.nf; f++) {
         const int k0 = c.nv[f];
-        V3 out[VMAX];
+        unsigned char out[VMAX];
         int no = 0;
         for (int k = 0; k < k0; k++) {
-            const V3 a = c.v[f][k];
-            const V3 b = c.v[f][(k + 1) % k0];
-            const double sa = dot3(n, a) - cc;
-            const double sb = dot3(n, b) - cc;
+            const int ia = c.ring[f][k];
+            const int ib = c.ring[f][(k + 1) % k0];
+            const double sa = s[ia];
+            const double sb = s[ib];
             if (sa <= tol) {
-                if (no < VMAX) out[no++] = a; else c.overflow = 1;
+                if (map[ia] == 255) {
+                    if (nn >= PMAX) { c.overflow = 1; return; }
+                    nvx[nn] = ov[ia];
+                    edge[nn] = 0xffff;
+                    map[ia] = (unsigned char)nn++;
+                }
+                if (no < VMAX) out[no++] = map[ia]; else c.overflow = 1;
                 if (sa >= -tol) {
-                    if (ncap < FMAX * 2) cap[ncap++] = a; else c.overflow = 1;
+                    if (ncap < FMAX * 2) cap[ncap++] = map[ia]; else c.overflow = 1;
                 }
             }
             if ((sa < -tol && sb > tol) || (sa > tol && sb < -tol)) {
-                const V3 u = sa < 0.0 ? a : b;
-                const V3 w = sa < 0.0 ? b : a;
-                const double su = sa < 0.0 ? sa : sb;
-                const double sw = sa < 0.0 ? sb : sa;
-                const double t = su / (su - sw);
-                const V3 x = {u.x + t * (w.x - u.x), u.y + t * (w.y - u.y), u.z + t * (w.z - u.z)};
-                if (no < VMAX) out[no++] = x; else c.overflow = 1;
-                if (ncap < FMAX * 2) cap[ncap++] = x; else c.overflow = 1;
+                const int iu = sa < 0.0 ? ia : ib;
+                const int iw = sa < 0.0 ? ib : ia;
+                const unsigned short key = (unsigned short)(iu << 8 | iw);
+                int ix = -1;
+                for (int t = 0; t < nn; t++)
+                    if (edge[t] == key) { ix = t; break; }
+                if (ix < 0) {
+                    if (nn >= PMAX) { c.overflow = 1; return; }
+                    const V3 u = ov[iu];
+                    const V3 w = ov[iw];
+                    const double su = s[iu];
+                    const double sw = s[iw];
+                    const double t = su / (su - sw);
+                    nvx[nn] = {u.x + t * (w.x - u.x), u.y + t * (w.y - u.y), u.z + t * (w.z - u.z)};
+                    edge[nn] = key;
+                    ix = nn++;
+                }
+                if (no < VMAX) out[no++] = (unsigned char)ix; else c.overflow = 1;
+                if (ncap < FMAX * 2) cap[ncap++] = (unsigned char)ix; else c.overflow = 1;
             }
         }
         if (no >= 3) {
             c.fid[nf] = c.fid[f];
             c.fh[nf] = c.fh[f];
-            c.nv[nf] = no;
-            for (int k = 0; k < no; k++) c.v[nf][k] = out[k];
+            c.nv[nf] = (unsigned char)no;
+            for (int k = 0; k < no; k++) c.ring[nf][k] = out[k];
             nf++;
         }
     }
-    V3 u[FMAX * 2];
+    c.cur ^= 1;
+    c.nvert = nn;
+    // distinct cap points (by value)
+    unsigned char u[VMAX];
     int nu = 0;
     for (int i = 0; i < ncap; i++) {
         bool dup = false;
         for (int j = 0; j < nu; j++)
-            if (cap[i].x == u[j].x && cap[i].y == u[j].y && cap[i].z == u[j].z) { dup = true; break; }
-        if (!dup) u[nu++] = cap[i];
+            if (same3(nvx[cap[i]], nvx[u[j]])) { dup = true; break; }
+        if (!dup) {
+            if (nu == VMAX) { c.overflow = 1; return; }   // cap face too large
+            u[nu++] = cap[i];
+        }
     }
     if (nu >= 3) {
         V3 m = {0.0, 0.0, 0.0};
-        for (int i = 0; i < nu; i++) { m.x += u[i].x; m.y += u[i].y; m.z += u[i].z; }
+        for (int i = 0; i < nu; i++) { m.x += nvx[u[i]].x; m.y += nvx[u[i]].y; m.z += nvx[u[i]].z; }
         m.x /= nu; m.y /= nu; m.z /= nu;
-        const V3 e1 = sub3(u[0], m);
+        const V3 e1 = sub3(nvx[u[0]], m);
         const V3 e2 = cross3(n, e1);
-        double pa[FMAX * 2], pb[FMAX * 2];
+        double pa[VMAX], pb[VMAX];
         for (int i = 0; i < nu; i++) {
-            const V3 d = sub3(u[i], m);
+            const V3 d = sub3(nvx[u[i]], m);
             pa[i] = dot3(d, e1);
             pb[i] = dot3(d, e2);
         }
-        for (int i = 1; i < nu; i++) {
-            const V3 tv = u[i];
+        for (int i = 1; i < nu; i++) {   // insertion sort by angle
+            const unsigned char tv = u[i];
             const double ta = pa[i], tb = pb[i];
             int j = i - 1;
             while (j >= 0 && ang_less(ta, tb, pa[j], pb[j])) {
@@ -143,11 +203,11 @@ __device__ void clip(Cell& c, const V3& n, int nid) {
             }
             u[j + 1] = tv; pa[j + 1] = ta; pb[j + 1] = tb;
         }
-        if (nf < FMAX && nu <= VMAX) {
+        if (nf < FMAX) {
             c.fid[nf] = nid;
             c.fh[nf] = sqrt(dot3(n, n));
-            c.nv[nf] = nu;
-            for (int k = 0; k < nu; k++) c.v[nf][k] = u[k];
+            c.nv[nf] = (unsigned char)nu;
+            for (int k = 0; k < nu; k++) c.ring[nf][k] = u[k];
             nf++;
         } else {
             c.overflow = 1;
@@ -157,7 +217,7 @@ __device__ void clip(Cell& c, const V3& n, int nid) {
     double r2 = 0.0;
     for (int f = 0; f < nf; f++)
         for (int k = 0; k < c.nv[f]; k++) {
-            const double d = dot3(c.v[f][k], c.v[f][k]);
+            const double d = dot3(nvx[c.ring[f][k]], nvx[c.ring[f][k]]);
             if (d > r2) r2 = d;
         }
     c.rmax2 = r2;
@@ -170,13 +230,14 @@ __device__ bool has_box_face(const Cell& c) {
 }
 
 __device__ double cell_volume(const Cell& c) {
+    const V3* v = c.v[c.cur];
     double vol = 0.0;
     for (int f = 0; f < c.nf; f++) {
         if (c.fid[f] < 0) return 0.0;
-        const V3 v0 = c.v[f][0];
+        const V3 v0 = v[c.ring[f][0]];
         double area = 0.0;
         for (int k = 1; k < c.nv[f] - 1; k++) {
-            const V3 cr = cross3(sub3(c.v[f][k], v0), sub3(c.v[f][k + 1], v0));
+            const V3 cr = cross3(sub3(v[c.ring[f][k]], v0), sub3(v[c.ring[f][k + 1]], v0));
             area += sqrt(dot3(cr, cr)) / 2.0;
         }
         vol += area * c.fh[f];
@@ -184,18 +245,18 @@ __device__ double cell_volume(const Cell& c) {
     return vol / 6.0;
 }
 
-__global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, long long np, VorHash H, double r0,
-                                               double* __restrict__ vol, int* __restrict__ overflow) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    Cell c;
+// Volume of point i's cell (0 when the cell keeps a box face); sets
+// c.overflow for a closed cell that outgrew the fixed limits.
+__device__ double point_volume(const double* __restrict__ p, long long i, const VorHash& H, Cell& c) {
     const V3 pi = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     int c3[3];
     for (int a = 0; a < 3; a++) {
         int k = (int)((p[3 * i + a] - H.lo[a]) / H.h);
         c3[a] = k < 0 ? 0 : (k >= H.n[a] ? H.n[a] - 1 : k);
     }
-    init_box(c, r0);
+    double blo[3], bhi[3];
+    for (int a = 0; a < 3; a++) { blo[a] = H.blo[a] - p[3 * i + a]; bhi[a] = H.bhi[a] - p[3 * i + a]; }
+    init_box(c, blo, bhi);
     for (int r = 0; r <= RMAX; r++) {
         for (int dx = -r; dx <= r; dx++)
             for (int dy = -r; dy <= r; dy++)
@@ -217,29 +278,40 @@ __global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, lo
                         const bool box = has_box_face(c);
                         clip(c, n, j);
                         if (c.overflow) {
-                            // a cell still reaching the bounding cube that outgrows the limits
+                            // a cell still reaching the bounding box that outgrows the limits
                             // is taken as unbounded (0); a closed cell's overflow stays an error
                             if (box) c.overflow = 0;
-                            vol[i] = 0.0;
-                            if (c.overflow) atomicAdd(overflow, 1);
-                            return;
+                            return 0.0;
                         }
                     }
                 }
         const double reach = (double)r * H.h;
         if (0.25 * reach * reach >= c.rmax2) break;
     }
-    vol[i] = cell_volume(c);
-    if (c.overflow) atomicAdd(overflow, 1);
+    return cell_volume(c);
+}
+
+// One cell per thread, grid-stride over the points.
+__global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, long long np, VorHash H,
+                                               double* __restrict__ vol, int* __restrict__ overflow) {
+    Cell c;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < np;
+         i += (long long)gridDim.x * blockDim.x) {
+        vol[i] = point_volume(p, i, H, c);
+        if (c.overflow) atomicAdd(overflow, 1);
+    }
 }
 
 }  // namespace vor
 
-hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double r0, double* vol, int* overflow,
+hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* overflow,
                           hipStream_t s) {
-    // 64 KB of dynamic LDS per block keeps at most two blocks per CU, which
-    // bounds the private (scratch) memory the runtime reserves for the cells.
-    vor::k_voronoi<<<(unsigned)((np + 63) / 64), 64, 65536, s>>>(pts, np, H, r0, vol, overflow);
+    // Each thread holds a cell of about 8.5 KB in private (scratch) memory, which
+    // the runtime backs for every wave of the launch.  The grid is capped at two
+    // blocks per CU (512 waves, about 300 MB of scratch) and 64 KB of dynamic
+    // LDS per block keeps at most two of them resident on a CU.
+    const long long blocks = std::min<long long>((np + 63) / 64, 512);
+    vor::k_voronoi<<<(unsigned)blocks, 64, 65536, s>>>(pts, np, H, vol, overflow);
     return hipGetLastError();
 }
 

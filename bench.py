@@ -60,6 +60,9 @@ def main():
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle accuracy check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tile", type=int, default=-1, help="gridding tile config (-1: autotune)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one device copy of 1 GiB (the reference kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -67,6 +70,15 @@ def main():
         sys.exit(1)
 
     import torch  # device-wide synchronisation around the timed region
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: %d bytes read, %d bytes written" % (src.numel() * 4, dst.numel() * 4))
+        del src, dst
 
     pkg = importlib.import_module("4d_lung_amd")
     synth = pkg.synth

@@ -8,13 +8,17 @@
 // exact IEEE operation sequence of the device kernel, so the device weights
 // can be checked bit for bit; voronoi_ref.py pins the geometry.
 //
-// Cell of point p: start from the cube |x - p|_inf <= R0 and intersect it with
+// Cell of point p: start from the bounding box of all points widened by one
+// hash cell and intersect it with
 // the half-spaces n.(x - p) <= |n|^2/2, n = q - p, of the other points q,
 // visiting hash cells shell by shell around p's cell (in a fixed order) until
 // the farthest cell vertex is within half the distance that any unvisited
-// point can have.  A cell that still has a cube face is unbounded (p on the
-// convex hull) and gets volume 0, as in volume_of_finite_dual_cell
-// (functions.h:148-150).  Volume = sum over faces of area * |q - p| / 6
+// point can have.  A cell that still has a face of the box is taken as
+// unbounded and gets volume 0, as points on the convex hull do in
+// volume_of_finite_dual_cell (functions.h:148-150); this also zeroes the
+// cells of interior points whose Voronoi region reaches more than one hash
+// cell beyond the point cloud (near-degenerate boundary cells whose CGAL
+// volume exceeds the whole k-space).  Volume = sum over faces of area * |q - p| / 6
 // (functions.h:156-158), face areas as triangle fans from the first vertex
 // (functions.h:117-124).
 #include <cmath>
@@ -25,22 +29,28 @@
 
 namespace vor {
 
-constexpr int FMAX = 48;
-constexpr int VMAX = 32;
+constexpr int FMAX = 48;   // faces of a cell
+constexpr int VMAX = 32;   // vertices of one face
+constexpr int PMAX = 96;   // distinct vertices of a cell (a simple polyhedron with F faces has <= 2F - 4)
 constexpr int RMAX = 10;   // hash shells visited at most
 
 struct V3 {
     double x, y, z;
 };
 
+// Faces are rings of indices into a vertex pool; clipping writes the new
+// pool into the other half of v[2][] and flips `cur`.
 struct Cell {
     int nf;
+    int nvert;
+    int cur;
+    int overflow;
+    double rmax2;
     int fid[FMAX];
     double fh[FMAX];
-    int nv[FMAX];
-    V3 v[FMAX][VMAX];
-    double rmax2;
-    int overflow;
+    unsigned char nv[FMAX];
+    unsigned char ring[FMAX][VMAX];
+    V3 v[2][PMAX];
 };
 
 inline double dot3(const V3& a, const V3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
@@ -49,20 +59,29 @@ inline V3 cross3(const V3& a, const V3& b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-void init_box(Cell& c, double r0) {
-    static const int q[6][4][3] = {
-        {{1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {1, -1, 1}},     {{-1, -1, -1}, {-1, -1, 1}, {-1, 1, 1}, {-1, 1, -1}},
-        {{-1, 1, -1}, {-1, 1, 1}, {1, 1, 1}, {1, 1, -1}},     {{-1, -1, -1}, {1, -1, -1}, {1, -1, 1}, {-1, -1, 1}},
-        {{-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}},     {{-1, -1, -1}, {-1, 1, -1}, {1, 1, -1}, {1, -1, -1}}};
+void init_box(Cell& c, const double* lo, const double* hi) {
+    // the bounding box [lo, hi] of all points (plus margin), relative to the
+    // cell's point; corner k has coordinates (bit 2, bit 1, bit 0) of k
+    const unsigned char q[6][4] = {{4, 6, 7, 5}, {0, 1, 3, 2}, {2, 3, 7, 6},
+                                   {0, 4, 5, 1}, {1, 5, 7, 3}, {0, 2, 6, 4}};
     c.nf = 6;
+    c.nvert = 8;
+    c.cur = 0;
     c.overflow = 0;
+    for (int k = 0; k < 8; k++)
+        c.v[0][k] = {(k & 4) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1], (k & 1) ? hi[2] : lo[2]};
+    double r2 = 0.0;
     for (int f = 0; f < 6; f++) {
         c.fid[f] = -1;
         c.fh[f] = 0.0;
         c.nv[f] = 4;
-        for (int k = 0; k < 4; k++) c.v[f][k] = {q[f][k][0] * r0, q[f][k][1] * r0, q[f][k][2] * r0};
+        for (int k = 0; k < 4; k++) {
+            c.ring[f][k] = q[f][k];
+            const double d = dot3(c.v[0][q[f][k]], c.v[0][q[f][k]]);
+            if (d > r2) r2 = d;
+        }
     }
-    c.rmax2 = 3.0 * r0 * r0;
+    c.rmax2 = r2;
 }
 
 // Angular order of cap points around their centroid in the plane with normal n.
@@ -73,74 +92,114 @@ inline bool ang_less(double a1, double b1, double a2, double b2) {
     return a1 * b2 - a2 * b1 > 0.0;
 }
 
+inline bool same3(const V3& a, const V3& b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+
+// Intersect the cell with the half-space n.x <= |n|^2/2.  Vertices with
+// signed distance within +-tol of the plane are kept and belong to the cap;
+// an edge whose ends lie strictly on both sides gets one new vertex,
+// interpolated from its inner end, shared by the two faces of the edge.  The
+// cap (the new face) is the set of distinct cap vertices, in first-seen order
+// over the faces, sorted by angle around their mean.
 void clip(Cell& c, const V3& n, int nid) {
     const double cc = 0.5 * dot3(n, n);
     const double tol = 1e-10 * cc;
+    const V3* __restrict__ ov = c.v[c.cur];
+    V3* __restrict__ nvx = c.v[c.cur ^ 1];
+    double s[PMAX];
+    for (int i = 0; i < c.nvert; i++) s[i] = dot3(n, ov[i]) - cc;
     bool any_out = false;
     for (int f = 0; f < c.nf && !any_out; f++)
         for (int k = 0; k < c.nv[f]; k++)
-            if (dot3(n, c.v[f][k]) - cc > tol) { any_out = true; break; }
+            if (s[c.ring[f][k]] > tol) { any_out = true; break; }
     if (!any_out) return;
-    V3 cap[FMAX * 2];
+    // new-pool index of every kept old vertex (255 = not yet placed), and for
+    // each new vertex the old edge it was cut from (0xffff for kept vertices)
+    unsigned char map[PMAX];
+    unsigned short edge[PMAX];
+    for (int i = 0; i < c.nvert; i++) map[i] = 255;
+    int nn = 0;
+    unsigned char cap[FMAX * 2];
     int ncap = 0;
     int nf = 0;
     for (int f = 0; f < c.nf; f++) {
         const int k0 = c.nv[f];
-        V3 out[VMAX];
+        unsigned char out[VMAX];
         int no = 0;
         for (int k = 0; k < k0; k++) {
-            const V3 a = c.v[f][k];
-            const V3 b = c.v[f][(k + 1) % k0];
-            const double sa = dot3(n, a) - cc;
-            const double sb = dot3(n, b) - cc;
+            const int ia = c.ring[f][k];
+            const int ib = c.ring[f][(k + 1) % k0];
+            const double sa = s[ia];
+            const double sb = s[ib];
             if (sa <= tol) {
-                if (no < VMAX) out[no++] = a; else c.overflow = 1;
+                if (map[ia] == 255) {
+                    if (nn >= PMAX) { c.overflow = 1; return; }
+                    nvx[nn] = ov[ia];
+                    edge[nn] = 0xffff;
+                    map[ia] = (unsigned char)nn++;
+                }
+                if (no < VMAX) out[no++] = map[ia]; else c.overflow = 1;
                 if (sa >= -tol) {
-                    if (ncap < FMAX * 2) cap[ncap++] = a; else c.overflow = 1;
+                    if (ncap < FMAX * 2) cap[ncap++] = map[ia]; else c.overflow = 1;
                 }
             }
             if ((sa < -tol && sb > tol) || (sa > tol && sb < -tol)) {
-                const V3 u = sa < 0.0 ? a : b;
-                const V3 w = sa < 0.0 ? b : a;
-                const double su = sa < 0.0 ? sa : sb;
-                const double sw = sa < 0.0 ? sb : sa;
-                const double t = su / (su - sw);
-                const V3 x = {u.x + t * (w.x - u.x), u.y + t * (w.y - u.y), u.z + t * (w.z - u.z)};
-                if (no < VMAX) out[no++] = x; else c.overflow = 1;
-                if (ncap < FMAX * 2) cap[ncap++] = x; else c.overflow = 1;
+                const int iu = sa < 0.0 ? ia : ib;
+                const int iw = sa < 0.0 ? ib : ia;
+                const unsigned short key = (unsigned short)(iu << 8 | iw);
+                int ix = -1;
+                for (int t = 0; t < nn; t++)
+                    if (edge[t] == key) { ix = t; break; }
+                if (ix < 0) {
+                    if (nn >= PMAX) { c.overflow = 1; return; }
+                    const V3 u = ov[iu];
+                    const V3 w = ov[iw];
+                    const double su = s[iu];
+                    const double sw = s[iw];
+                    const double t = su / (su - sw);
+                    nvx[nn] = {u.x + t * (w.x - u.x), u.y + t * (w.y - u.y), u.z + t * (w.z - u.z)};
+                    edge[nn] = key;
+                    ix = nn++;
+                }
+                if (no < VMAX) out[no++] = (unsigned char)ix; else c.overflow = 1;
+                if (ncap < FMAX * 2) cap[ncap++] = (unsigned char)ix; else c.overflow = 1;
             }
         }
         if (no >= 3) {
             c.fid[nf] = c.fid[f];
             c.fh[nf] = c.fh[f];
-            c.nv[nf] = no;
-            for (int k = 0; k < no; k++) c.v[nf][k] = out[k];
+            c.nv[nf] = (unsigned char)no;
+            for (int k = 0; k < no; k++) c.ring[nf][k] = out[k];
             nf++;
         }
     }
-    // distinct cap points
-    V3 u[FMAX * 2];
+    c.cur ^= 1;
+    c.nvert = nn;
+    // distinct cap points (by value)
+    unsigned char u[VMAX];
     int nu = 0;
     for (int i = 0; i < ncap; i++) {
         bool dup = false;
         for (int j = 0; j < nu; j++)
-            if (cap[i].x == u[j].x && cap[i].y == u[j].y && cap[i].z == u[j].z) { dup = true; break; }
-        if (!dup) u[nu++] = cap[i];
+            if (same3(nvx[cap[i]], nvx[u[j]])) { dup = true; break; }
+        if (!dup) {
+            if (nu == VMAX) { c.overflow = 1; return; }   // cap face too large
+            u[nu++] = cap[i];
+        }
     }
     if (nu >= 3) {
         V3 m = {0.0, 0.0, 0.0};
-        for (int i = 0; i < nu; i++) { m.x += u[i].x; m.y += u[i].y; m.z += u[i].z; }
+        for (int i = 0; i < nu; i++) { m.x += nvx[u[i]].x; m.y += nvx[u[i]].y; m.z += nvx[u[i]].z; }
         m.x /= nu; m.y /= nu; m.z /= nu;
-        const V3 e1 = sub3(u[0], m);
+        const V3 e1 = sub3(nvx[u[0]], m);
         const V3 e2 = cross3(n, e1);
-        double pa[FMAX * 2], pb[FMAX * 2];
+        double pa[VMAX], pb[VMAX];
         for (int i = 0; i < nu; i++) {
-            const V3 d = sub3(u[i], m);
+            const V3 d = sub3(nvx[u[i]], m);
             pa[i] = dot3(d, e1);
             pb[i] = dot3(d, e2);
         }
         for (int i = 1; i < nu; i++) {   // insertion sort by angle
-            const V3 tv = u[i];
+            const unsigned char tv = u[i];
             const double ta = pa[i], tb = pb[i];
             int j = i - 1;
             while (j >= 0 && ang_less(ta, tb, pa[j], pb[j])) {
@@ -149,11 +208,11 @@ void clip(Cell& c, const V3& n, int nid) {
             }
             u[j + 1] = tv; pa[j + 1] = ta; pb[j + 1] = tb;
         }
-        if (nf < FMAX && nu <= VMAX) {
+        if (nf < FMAX) {
             c.fid[nf] = nid;
             c.fh[nf] = std::sqrt(dot3(n, n));
-            c.nv[nf] = nu;
-            for (int k = 0; k < nu; k++) c.v[nf][k] = u[k];
+            c.nv[nf] = (unsigned char)nu;
+            for (int k = 0; k < nu; k++) c.ring[nf][k] = u[k];
             nf++;
         } else {
             c.overflow = 1;
@@ -163,7 +222,7 @@ void clip(Cell& c, const V3& n, int nid) {
     double r2 = 0.0;
     for (int f = 0; f < nf; f++)
         for (int k = 0; k < c.nv[f]; k++) {
-            const double d = dot3(c.v[f][k], c.v[f][k]);
+            const double d = dot3(nvx[c.ring[f][k]], nvx[c.ring[f][k]]);
             if (d > r2) r2 = d;
         }
     c.rmax2 = r2;
@@ -176,13 +235,14 @@ bool has_box_face(const Cell& c) {
 }
 
 double cell_volume(const Cell& c) {
+    const V3* v = c.v[c.cur];
     double vol = 0.0;
     for (int f = 0; f < c.nf; f++) {
         if (c.fid[f] < 0) return 0.0;
-        const V3 v0 = c.v[f][0];
+        const V3 v0 = v[c.ring[f][0]];
         double area = 0.0;
         for (int k = 1; k < c.nv[f] - 1; k++) {
-            const V3 cr = cross3(sub3(c.v[f][k], v0), sub3(c.v[f][k + 1], v0));
+            const V3 cr = cross3(sub3(v[c.ring[f][k]], v0), sub3(v[c.ring[f][k + 1]], v0));
             area += std::sqrt(dot3(cr, cr)) / 2.0;
         }
         vol += area * c.fh[f];
@@ -192,6 +252,7 @@ double cell_volume(const Cell& c) {
 
 struct Hash {
     double lo[3];
+    double blo[3], bhi[3];   // bounding box of the points widened by one hash cell
     double h;
     int n[3];
     std::vector<int> start, idx;
@@ -210,6 +271,7 @@ Hash build_hash(const double* p, long np) {
     for (int a = 0; a < 3; a++) vol *= std::fmax(hi[a] - H.lo[a], 1e-12);
     H.h = std::cbrt(vol * 6.0 / (double)np);
     for (int a = 0; a < 3; a++) H.n[a] = (int)((hi[a] - H.lo[a]) / H.h) + 1;
+    for (int a = 0; a < 3; a++) { H.blo[a] = H.lo[a] - H.h; H.bhi[a] = hi[a] + H.h; }
     const long nc = (long)H.n[0] * H.n[1] * H.n[2];
     std::vector<int> cnt(nc + 1, 0), key(np);
     for (long i = 0; i < np; i++) {
@@ -229,14 +291,16 @@ Hash build_hash(const double* p, long np) {
     return H;
 }
 
-double point_volume(const double* p, long i, const Hash& H, double r0, Cell& c) {
+double point_volume(const double* p, long i, const Hash& H, Cell& c) {
     const V3 pi = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     int c3[3];
     for (int a = 0; a < 3; a++) {
         int k = (int)((p[3 * i + a] - H.lo[a]) / H.h);
         c3[a] = k < 0 ? 0 : (k >= H.n[a] ? H.n[a] - 1 : k);
     }
-    init_box(c, r0);
+    double blo[3], bhi[3];
+    for (int a = 0; a < 3; a++) { blo[a] = H.blo[a] - p[3 * i + a]; bhi[a] = H.bhi[a] - p[3 * i + a]; }
+    init_box(c, blo, bhi);
     for (int r = 0; r <= RMAX; r++) {
         for (int dx = -r; dx <= r; dx++)
             for (int dy = -r; dy <= r; dy++)
@@ -279,16 +343,13 @@ extern "C" {
 // returns the number of cells that overflowed the fixed face/vertex limits.
 long orc_voronoi_volumes(const double* pts, long np, double* vol, int nthreads) {
     vor::Hash H = vor::build_hash(pts, np);
-    double ext = 0.0;
-    for (int a = 0; a < 3; a++) ext = std::fmax(ext, H.h * H.n[a]);
-    const double r0 = 4.0 * ext;
     long over = 0;
 #pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads()) reduction(+ : over)
     {
         vor::Cell* c = new vor::Cell;
 #pragma omp for schedule(dynamic, 256)
         for (long i = 0; i < np; i++) {
-            vol[i] = vor::point_volume(pts, i, H, r0, *c);
+            vol[i] = vor::point_volume(pts, i, H, *c);
             over += c->overflow;
         }
         delete c;

@@ -38,9 +38,11 @@ def test_clipping_matches_qhull_dual_sum(orc):
     b = voronoi_ref.voronoi_volumes(p)
     both = (a > 0) & (b > 0)
     np.testing.assert_allclose(a[both], b[both], rtol=1e-10)
-    # cells the shell search leaves unbounded lie in a thin layer at the hull
-    assert np.all(b[a == 0] == 0) or np.all(np.min(np.minimum(p, 1 - p)[(a == 0) & (b > 0)], 1) < 0.2)
-    assert (a == 0).sum() - (b == 0).sum() < 0.05 * len(p)
+    # every hull point (unbounded cell) is 0 here too; the extra zeros are the
+    # near-boundary cells that reach beyond the widened bounding box
+    assert np.all(a[b == 0] == 0)
+    extra = (a == 0) & (b > 0)
+    assert np.all(np.min(np.minimum(p, 1 - p)[extra], 1) < 0.15)
 
 
 def test_wave_caipi_interior_matches_qhull(orc):
@@ -53,7 +55,8 @@ def test_wave_caipi_interior_matches_qhull(orc):
     a = orc.voronoi_volumes(p)
     b = voronoi_ref.voronoi_volumes(p)
     both = (a > 0) & (b > 0)
-    assert both.sum() > 0.8 * len(p)
+    assert both.sum() > 0.85 * len(p)
+    assert np.all(a[b == 0] == 0)
     np.testing.assert_allclose(a[both], b[both], rtol=1e-10)
 
 
