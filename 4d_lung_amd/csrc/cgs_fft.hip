@@ -262,29 +262,70 @@ __host__ __device__ constexpr int plan_ls(int L, int S) {
     return ls;
 }
 
+// Last stage (LS * R = L) of a tile held in LDS: butterfly j reads lines
+// j + i * LS, and its outputs -- lines j + i * LS again -- go to st(l, col, v)
+// instead of back to LDS.  With LDS_STORE a barrier separates the LDS reads
+// from the stores, so that st may write the tile.  Also serves L with a single
+// stage (LS = 1).
+template <int L, int DIR, int NCOL, int STRIDE, bool LDS_STORE, int NT, class Store>
+__device__ __forceinline__ void fft_last_stage(const float2* lds, const float2* tw, Store st) {
+    constexpr int NS = plan_nstages(L);
+    constexpr int R = plan_radix(L, NS - 1);
+    constexpr int LS = plan_ls(L, NS - 1);
+    constexpr int NB = NCOL * (L / R);
+    constexpr int PER = (NB + NT - 1) / NT;
+    const int tid = threadIdx.x;
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = tid + p * NT;
+        if ((NB % NT) == 0 || b < NB) {
+            const int col = b % NCOL, j = b / NCOL;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                float2 x = lds[(j + i * LS) * STRIDE + col];
+                if (LS > 1 && i > 0) {
+                    float2 t = tw[j * i];
+                    if (DIR > 0) t.y = -t.y;
+                    x = cmul(x, t);
+                }
+                v[p][i] = x;
+            }
+        }
+    }
+    if constexpr (LDS_STORE) __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = tid + p * NT;
+        if ((NB % NT) == 0 || b < NB) {
+            const int col = b % NCOL, j = b / NCOL;
+            Dft<R, DIR>::run(v[p]);
+#pragma unroll
+            for (int i = 0; i < R; ++i) st(j + i * LS, col, v[p][i]);
+        }
+    }
+}
+
+// Line transform of an [L x NCOL] tile already in LDS (and visible to the
+// block): every stage but the last through LDS, the last into st.
+template <int L, int DIR, int NCOL, int STRIDE, bool LDS_STORE, int NT, class Store>
+__device__ __forceinline__ void fft_lds_tile(float2* lds, const float2* tw, Store st) {
+    constexpr int NS = plan_nstages(L);
+    fft_mid_stages<L, 0, NS - 1, 1, DIR, NCOL, STRIDE, NT>(lds, tw);
+    fft_last_stage<L, DIR, NCOL, STRIDE, LDS_STORE, NT>(lds, tw, st);
+}
+
 // Whole line transform of an [L x NCOL] tile whose first stage takes its
 // inputs straight from ld(l, col) (HBM -> registers) and whose last stage
 // hands its outputs straight to st(l, col, v) (registers -> HBM or an
 // epilogue); only the stages in between go through LDS.  tw: L twiddles
-// exp(-2 pi i m / L), visible to the block after the first barrier.  With
-// LDS_STORE the last stage waits at a barrier between its LDS reads and the
-// stores, so that st may write the tile.  Ends without a barrier.
-// With FUSED_FIRST false the tile is first loaded into LDS element by element
-// (consecutive threads on consecutive columns) and every stage but the last
-// runs through LDS; this needs fewer registers when ld is expensive.
-template <int L, int DIR, int NCOL, int STRIDE, bool LDS_STORE, bool FUSED_FIRST = true, int NT = CGS_THREADS,
-          class Load, class Store>
+// exp(-2 pi i m / L), visible to the block after the first barrier.  Ends
+// without a barrier.
+template <int L, int DIR, int NCOL, int STRIDE, bool LDS_STORE, int NT, class Load, class Store>
 __device__ __forceinline__ void fft_line_tile(float2* lds, const float2* tw, Load ld, Store st) {
     constexpr int NS = plan_nstages(L);
     const int tid = threadIdx.x;
-    if constexpr (!FUSED_FIRST && NS > 1) {
-        for (int e = tid; e < L * NCOL; e += NT) {
-            const int c = e % NCOL, l = e / NCOL;
-            lds[l * STRIDE + c] = ld(l, c);
-        }
-        __syncthreads();
-        fft_mid_stages<L, 0, NS - 1, 1, DIR, NCOL, STRIDE, NT>(lds, tw);
-    } else if constexpr (NS == 1) {
+    if constexpr (NS == 1) {
         constexpr int R = L;
         constexpr int PER = (NCOL + NT - 1) / NT;
 #pragma unroll
@@ -327,84 +368,168 @@ __device__ __forceinline__ void fft_line_tile(float2* lds, const float2* tw, Loa
             __syncthreads();
         }
         fft_mid_stages<L, 1, NS - 1, plan_radix(L, 0), DIR, NCOL, STRIDE, NT>(lds, tw);
+        fft_last_stage<L, DIR, NCOL, STRIDE, LDS_STORE, NT>(lds, tw, st);
     }
-    if constexpr (NS > 1) {
-        {   // last stage: LS * R = L, output line index k + i * LS with k = j
-            constexpr int R = plan_radix(L, NS - 1);
-            constexpr int LS = plan_ls(L, NS - 1);
-            constexpr int NB = NCOL * (L / R);
-            constexpr int PER = (NB + NT - 1) / NT;
-            float2 v[PER][R];
+}
+
+// A thread's share of an [L x NCOL] tile in registers, element e = tid + k*NT
+// at line e / NCOL, column e % NCOL (consecutive threads on consecutive
+// columns, so each line is one coalesced run).  Holds the raw loads (type T)
+// of the next tile while the current one is transformed; cv(l, col, raw)
+// turns them into tile values when they go to LDS, after the loads landed.
+template <int L, int NCOL, int NT, class T>
+struct TileRegs {
+    static constexpr int EPT = (L * NCOL + NT - 1) / NT;
+    T v[EPT];
+    template <class Load>
+    __device__ __forceinline__ void load(Load ld) {
 #pragma unroll
-            for (int p = 0; p < PER; ++p) {
-                const int b = tid + p * NT;
-                if ((NB % NT) == 0 || b < NB) {
-                    const int col = b % NCOL, j = b / NCOL;
+        for (int k = 0; k < EPT; ++k) {
+            const int e = threadIdx.x + k * NT;
+            if ((L * NCOL) % NT == 0 || e < L * NCOL) v[k] = ld(e / NCOL, e % NCOL);
+        }
+    }
+    template <int STRIDE, class Conv>
+    __device__ __forceinline__ void to_lds(float2* lds, Conv cv) const {
 #pragma unroll
-                    for (int i = 0; i < R; ++i) {
-                        float2 x = lds[(j + i * LS) * STRIDE + col];
-                        if (i > 0) {
-                            float2 t = tw[j * i];
-                            if (DIR > 0) t.y = -t.y;
-                            x = cmul(x, t);
-                        }
-                        v[p][i] = x;
-                    }
-                }
-            }
-            if constexpr (LDS_STORE) __syncthreads();
-#pragma unroll
-            for (int p = 0; p < PER; ++p) {
-                const int b = tid + p * NT;
-                if ((NB % NT) == 0 || b < NB) {
-                    const int col = b % NCOL, j = b / NCOL;
-                    Dft<R, DIR>::run(v[p]);
-#pragma unroll
-                    for (int i = 0; i < R; ++i) st(j + i * LS, col, v[p][i]);
-                }
+        for (int k = 0; k < EPT; ++k) {
+            const int e = threadIdx.x + k * NT;
+            if ((L * NCOL) % NT == 0 || e < L * NCOL) {
+                const int l = e / NCOL, c = e % NCOL;
+                lds[l * STRIDE + c] = cv(l, c, v[k]);
             }
         }
     }
-}
+};
+
+// The first stage's inputs of a thread (butterfly b = tid + p*NT reads lines
+// j + i*L/R0 of column b % NCOL, j = b / NCOL) held in registers, so that the
+// next tile can be fetched while the current one is transformed; run() does
+// the radix-R0 butterflies and writes the stage output to LDS.
+template <int L, int NCOL, int NT>
+struct FirstRegs {
+    static constexpr int R = plan_radix(L, 0);
+    static constexpr int NB = NCOL * (L / R);
+    static constexpr int PER = (NB + NT - 1) / NT;
+    float2 v[PER][R];
+    template <class Load>
+    __device__ __forceinline__ void load(Load ld) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int b = threadIdx.x + p * NT;
+            if ((NB % NT) == 0 || b < NB) {
+                const int col = b % NCOL, j = b / NCOL;
+#pragma unroll
+                for (int i = 0; i < R; ++i) v[p][i] = ld(j + i * (L / R), col);
+            }
+        }
+    }
+    template <int DIR, int STRIDE, class Conv>
+    __device__ __forceinline__ void run(float2* lds, Conv cv) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int b = threadIdx.x + p * NT;
+            if ((NB % NT) == 0 || b < NB) {
+                const int col = b % NCOL, j = b / NCOL;
+#pragma unroll
+                for (int i = 0; i < R; ++i) v[p][i] = cv(j + i * (L / R), col, v[p][i]);
+                Dft<R, DIR>::run(v[p]);
+#pragma unroll
+                for (int i = 0; i < R; ++i) lds[(j * R + i) * STRIDE + col] = v[p][i];
+            }
+        }
+    }
+};
+
+// Raw inputs of one forward z-pass element: u * w * k (w = 1, k = 1 where
+// the product has fewer factors).
+struct ZRaw {
+    float2 u, w;
+    float k;
+};
 
 __device__ __forceinline__ float2 alt_sign(float2 v, int l) {
     return (l & 1) ? make_float2(-v.x, -v.y) : v;
 }
 
 // ------------------------------------------------------------ pass kernels
-// First stage of the z passes: the forward pass (three loads per element)
-// stages its tile through LDS, the inverse pass loads into registers.
-#ifndef CGS_ZPRE_FUSED_FIRST
-#define CGS_ZPRE_FUSED_FIRST false
-#endif
-#ifndef CGS_ZPOST_FUSED_FIRST
-#define CGS_ZPOST_FUSED_FIRST true
-#endif
 
-// Strided pass (y or x axis): block = one outer index x one chunk of W
-// consecutive columns; lines of length L with stride a.ls.
-template <int L, int W, int DIR>
+// Strided pass (y or x axis): block = one outer index x CPB consecutive
+// chunks of W columns; lines of length L with stride a.ls.  The first-stage
+// inputs of the next chunk are fetched while the current chunk is transformed.
+#ifndef CGS_PLAIN_CPB
+#define CGS_PLAIN_CPB 2
+#endif
+// Element at a 32-bit byte offset (or a 64-bit element offset) from a
+// block-uniform base: the 32-bit form lets the load use a scalar base address
+// plus one offset register.
+template <class Off>
+__device__ __forceinline__ float2* elem(float2* base, Off off) {
+    if constexpr (sizeof(Off) == 4) return (float2*)((char*)base + (uint32_t)(off * (Off)sizeof(float2)));
+    else return base + off;
+}
+
+template <int L, int W, int DIR, int CPB, class Off>
 __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
-    // LDS holds only the tile (it bounds the blocks per CU here); twiddles
-    // are read through the L1 cache
+    // Off: type of the element offsets from the chunk's first column (32-bit
+    // where every byte offset of the line group fits in 32 bits)
     __shared__ float2 lds[L * W];
+    __shared__ float2 twl[L];
     const long long nchunk = (a.ncols + W - 1) / W;
+    const long long ngrp = (nchunk + CPB - 1) / CPB;
     const long long bid = blockIdx.x;
-    const long long chunk = bid % nchunk, outer = bid / nchunk;
+    const long long grp = bid % ngrp, outer = bid / ngrp;
     if (a.done && a.done[outer / a.outer_per_bin]) return;
-    float2* g = a.g + outer * a.os + chunk * W;
-    const long long ls = a.ls;
-    const int ncol_here = (int)min((long long)W, a.ncols - chunk * W);
-    fft_line_tile<L, DIR, W, W, false>(
-        lds, a.tw,
-        [&](int l, int c) {
-            float2 v = make_float2(0.0f, 0.0f);
-            if (c < ncol_here) v = g[(long long)l * ls + c];
-            return alt_sign(v, l);
-        },
-        [&](int l, int c, float2 v) {
-            if (c < ncol_here) g[(long long)l * ls + c] = alt_sign(v, l);
+    for (int e = threadIdx.x; e < L; e += CGS_THREADS) twl[e] = a.tw[e];
+    float2* gout = a.g + outer * a.os;
+    const Off ls = (Off)a.ls;
+    const long long ncols = a.ncols;
+    constexpr int NS = plan_nstages(L);
+    const long long c_first = grp * CPB;
+    const long long c_end = min(c_first + CPB, nchunk);
+    if constexpr (NS == 1) {   // one radix stage: registers only
+        for (long long chunk = c_first; chunk < c_end; ++chunk) {
+            float2* g = gout + chunk * W;
+            const long long nh = ncols - chunk * W;
+            fft_line_tile<L, DIR, W, W, false, CGS_THREADS>(
+                lds, twl,
+                [&](int l, int c) {
+                    float2 v = make_float2(0.0f, 0.0f);
+                    if (W <= nh || c < nh) v = *elem<Off>(g, (Off)l * ls + (Off)c);
+                    return alt_sign(v, l);
+                },
+                [&](int l, int c, float2 v) {
+                    if (W <= nh || c < nh) *elem<Off>(g, (Off)l * ls + (Off)c) = alt_sign(v, l);
+                });
+        }
+        return;
+    }
+    // columns past the end of the last chunk: loads clamped to its last
+    // column (unconditional), values zeroed in registers, stores skipped
+    int cmax = 0;
+    auto cv = [&](int l, int c, float2 v) {
+        if (c > cmax) v = make_float2(0.0f, 0.0f);
+        return alt_sign(v, l);
+    };
+    auto load_chunk = [&](FirstRegs<L, W, CGS_THREADS>& r, long long chunk) {
+        float2* g = gout + chunk * W;
+        const int cm = (int)min((long long)W, ncols - chunk * W) - 1;
+        r.load([&](int l, int c) { return *elem<Off>(g, (Off)l * ls + (Off)min(c, cm)); });
+    };
+    FirstRegs<L, W, CGS_THREADS> next;
+    load_chunk(next, c_first);
+    for (long long chunk = c_first; chunk < c_end; ++chunk) {
+        cmax = (int)min((long long)W, ncols - chunk * W) - 1;
+        next.template run<DIR, W>(lds, cv);
+        __syncthreads();
+        if (chunk + 1 < c_end) load_chunk(next, chunk + 1);
+        fft_mid_stages<L, 1, NS - 1, plan_radix(L, 0), DIR, W, W, CGS_THREADS>(lds, twl);
+        float2* g = gout + chunk * W;
+        fft_last_stage<L, DIR, W, W, false, CGS_THREADS>(lds, twl, [&](int l, int c, float2 v) {
+            if (c <= cmax) *elem<Off>(g, (Off)l * ls + (Off)c) = alt_sign(v, l);
         });
+        if (chunk + 1 < c_end) __syncthreads();
+    }
 }
 
 // z pass with a single column (one complex grid, C = 1): block = OB lines.
@@ -426,12 +551,13 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_z1(float2* g, long long nli
     }
 }
 
-// Forward z pass of E with the fused prologue.  Block = one (bin, x, y) line
-// group; the block walks the coils in chunks of W.
+// Forward z pass of E with the fused prologue.  Block = one (bin, x, y) line;
+// the block walks the coils in chunks of W, fetching the next chunk into
+// registers while the current one is transformed.
 //   a.p != null : value = p[bin][cell] * S[cell][c]      (elementProduct, Reco.cpp:85-93)
 //   a.p == null : value = G[bin][cell][c]                 (coil image given)
 //   a.kft != null: value *= kft[cell]                     (deapodization, Reco.cpp:529-535)
-template <int L, int W, int NT>
+template <int L, int W, int NT, bool HAS_P, bool HAS_K>
 __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     __shared__ float2 lds[L * W];
     __shared__ float2 twl[L];
@@ -440,31 +566,52 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     if (a.done && a.done[bin]) return;
     for (int e = threadIdx.x; e < L; e += NT) twl[e] = a.tw[e];
     const long long cell0 = xy * L;                  // cell index of (x, y, z=0)
+    // block-uniform bases and 32-bit offsets within the line; coil indices
+    // are clamped so that every load is unconditional (columns past the last
+    // coil are zeroed when the tile is written to LDS)
     float2* g = a.g + (bin * a.ng + cell0) * a.C;
     const int C = a.C;
+    const float2* pl = HAS_P ? a.p + bin * a.ng + cell0 : nullptr;
+    const float2* ml = HAS_P ? a.maps + cell0 * C : nullptr;
+    const float* kl = HAS_K ? a.kft + cell0 : nullptr;
+    auto ld = [&](int c0) {
+        return [&, c0](int l, int c) {
+            const int coil = min(c0 + c, C - 1);
+            ZRaw r;
+            if constexpr (HAS_P) {
+                r.u = pl[l];
+                r.w = ml[l * C + coil];
+            } else {
+                r.u = g[l * C + coil];
+            }
+            if constexpr (HAS_K) r.k = kl[l];
+            return r;
+        };
+    };
+    int cur = 0;
+    auto cv = [&](int l, int c, const ZRaw& r) {
+        float2 v = HAS_P ? cmul(r.u, r.w) : r.u;
+        if constexpr (HAS_K) v = cscale(v, r.k);
+        if (cur + c >= C) v = make_float2(0.0f, 0.0f);
+        return alt_sign(v, l);
+    };
+    TileRegs<L, W, NT, ZRaw> next;
+    next.load(ld(0));
     for (int c0 = 0; c0 < C; c0 += W) {
-        fft_line_tile<L, -1, W, W, false, CGS_ZPRE_FUSED_FIRST, NT>(
-            lds, twl,
-            [&](int l, int c) {
-                const int coil = c0 + c;
-                float2 v = make_float2(0.0f, 0.0f);
-                if (coil < C) {
-                    const long long cell = cell0 + l;
-                    if (a.p) v = cmul(a.p[bin * a.ng + cell], a.maps[cell * C + coil]);
-                    else v = g[(long long)l * C + coil];
-                    if (a.kft) v = cscale(v, a.kft[cell]);
-                }
-                return alt_sign(v, l);
-            },
-            [&](int l, int c, float2 v) {
-                const int coil = c0 + c;
-                if (coil < C) g[(long long)l * C + coil] = alt_sign(v, l);
-            });
+        cur = c0;
+        next.template to_lds<W>(lds, cv);
+        __syncthreads();
+        if (c0 + W < C) next.load(ld(c0 + W));
+        fft_lds_tile<L, -1, W, W, false, NT>(lds, twl, [&](int l, int c, float2 v) {
+            const int coil = c0 + c;
+            if (coil < C) g[l * C + coil] = alt_sign(v, l);
+        });
         if (c0 + W < C) __syncthreads();
     }
 }
 
-// Inverse z pass of E^H with the fused epilogue.  Block = one (bin, x, y).
+// Inverse z pass of E^H with the fused epilogue.  Block = one (bin, x, y);
+// coil chunks of W, the next one fetched while the current one is transformed.
 // After the inverse transform each coil value is scaled by kft[cell]*a.scale
 // (IFFT 1/sqrt(Ng) and deapodization *1e-3, Reco.cpp:506-515) and cropped to
 // the reduceFOV box (Reco.cpp:626-671, inclusive upper bound).
@@ -491,35 +638,42 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
     float2* g = a.g + (bin * a.ng + cell0) * a.C;
     const bool combine = a.maps != nullptr;
     const int C = a.C;
+    const float2* ml = a.maps ? a.maps + cell0 * C : nullptr;
+    const float* kl = a.kft ? a.kft + cell0 : nullptr;
     float2 acc[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) acc[r] = make_float2(0.0f, 0.0f);
+    auto ld = [&](int c0) {
+        return [&, c0](int l, int c) { return g[l * C + min(c0 + c, C - 1)]; };
+    };
+    int cur = 0;
+    auto cv = [&](int l, int c, float2 v) {
+        if (cur + c >= C) v = make_float2(0.0f, 0.0f);
+        return alt_sign(v, l);
+    };
+    TileRegs<L, W, NT, float2> next;
+    next.load(ld(0));
     for (int c0 = 0; c0 < C; c0 += W) {
+        cur = c0;
+        next.template to_lds<LDW>(lds, cv);
+        __syncthreads();
+        if (c0 + W < C) next.load(ld(c0 + W));
         // inverse transform; the epilogue runs on the last stage's outputs:
         // products with conj(S) go back into the tile for the coil sum, coil
         // images go straight to the grid
-        fft_line_tile<L, +1, W, LDW, true, CGS_ZPOST_FUSED_FIRST, NT>(
-            lds, twl,
-            [&](int l, int c) {
-                const int coil = c0 + c;
-                float2 v = make_float2(0.0f, 0.0f);
-                if (coil < C) v = g[(long long)l * C + coil];
-                return alt_sign(v, l);
-            },
-            [&](int l, int c, float2 v) {
-                const int coil = c0 + c;
-                const long long cell = cell0 + l;
-                float2 r = make_float2(0.0f, 0.0f);
-                if (coil < C) {
-                    r = alt_sign(v, l);
-                    const float sc = a.kft ? a.kft[cell] * a.scale : a.scale;
-                    r = cscale(r, sc);
-                    if (!xy_in || (a.crop && (l < a.lo[2] || l > a.hi[2]))) r = make_float2(0.0f, 0.0f);
-                    if (combine) r = cmulc(a.maps[cell * C + coil], r);
-                    else g[(long long)l * C + coil] = r;
-                }
-                if (combine) lds[l * LDW + c] = r;
-            });
+        fft_lds_tile<L, +1, W, LDW, true, NT>(lds, twl, [&](int l, int c, float2 v) {
+            const int coil = c0 + c;
+            float2 r = make_float2(0.0f, 0.0f);
+            if (coil < C) {
+                r = alt_sign(v, l);
+                const float sc = kl ? kl[l] * a.scale : a.scale;
+                r = cscale(r, sc);
+                if (!xy_in || (a.crop && (l < a.lo[2] || l > a.hi[2]))) r = make_float2(0.0f, 0.0f);
+                if (combine) r = cmulc(ml[l * C + coil], r);
+                else g[l * C + coil] = r;
+            }
+            if (combine) lds[l * LDW + c] = r;
+        });
         if (combine) {
             __syncthreads();
 #pragma unroll
@@ -581,8 +735,14 @@ bool fft_len_supported(int L) {
     }
 }
 
-// Column chunk of the strided passes: keep the LDS tile at or below 32K complex.
-static inline int plain_w(int L) { return L <= 256 ? 32 : 16; }
+// Column chunk of the strided passes.
+#ifndef CGS_PLAIN_WS
+#define CGS_PLAIN_WS 32
+#endif
+#ifndef CGS_PLAIN_WB
+#define CGS_PLAIN_WB 16
+#endif
+static constexpr int plain_w(int L) { return L <= 256 ? CGS_PLAIN_WS : CGS_PLAIN_WB; }
 
 // Coil chunk of the z pass: 8 coils per LDS tile (small tiles keep four or
 // more blocks resident per CU, which hides the HBM latency better than wider
@@ -590,23 +750,23 @@ static inline int plain_w(int L) { return L <= 256 ? 32 : 16; }
 int zpass_w(int L, int C) {
     (void)L;
     (void)C;
-#ifdef CGS_ZW
-    return CGS_ZW;
-#else
     return 8;
-#endif
 }
 
 template <int L, int DIR>
 static void plain_launch(const PlainArgs& a, dim3 grid, hipStream_t s) {
-    if constexpr (L <= 256) k_fft_plain<L, 32, DIR><<<grid, CGS_THREADS, 0, s>>>(a);
-    else k_fft_plain<L, 16, DIR><<<grid, CGS_THREADS, 0, s>>>(a);
+    constexpr int W = plain_w(L);
+    // largest byte offset from a chunk's first column
+    const long long span = ((long long)(L - 1) * a.ls + a.ncols) * (long long)sizeof(float2);
+    if (span < (1LL << 32)) k_fft_plain<L, W, DIR, CGS_PLAIN_CPB, uint32_t><<<grid, CGS_THREADS, 0, s>>>(a);
+    else k_fft_plain<L, W, DIR, CGS_PLAIN_CPB, long long><<<grid, CGS_THREADS, 0, s>>>(a);
 }
 
 hipError_t launch_fft_plain(int L, int dir, const PlainArgs& a, long long nouter, hipStream_t s) {
     const int W = plain_w(L);
     const long long nchunk = (a.ncols + W - 1) / W;
-    const dim3 grid((unsigned)(nouter * nchunk));
+    const long long ngrp = (nchunk + CGS_PLAIN_CPB - 1) / CGS_PLAIN_CPB;
+    const dim3 grid((unsigned)(nouter * ngrp));
     switch (L) {
 #define CGS_CASE(n)                                            \
     case n:                                                    \
@@ -636,32 +796,24 @@ hipError_t launch_fft_z1(int L, int dir, float2* g, long long nlines, const floa
     return hipGetLastError();
 }
 
-// z-pass block shape: W coils per LDS tile, NT threads (16-coil tiles use
-// 512 threads so that each thread keeps the register footprint of the
-// 8-coil, 256-thread shape).
+
+// z-pass block shape: 8 coils per LDS tile, 256 threads.
 template <int L>
 static hipError_t zpre_w(int W, const ZArgs& a, long long nouter, hipStream_t s) {
     const dim3 grid((unsigned)nouter);
-    if (W == 8) {
-        k_fft_zpre<L, 8, 256><<<grid, 256, 0, s>>>(a);
-    } else if (W == 16 && L * 17 <= 8192) {
-        if constexpr (L * 17 <= 8192) k_fft_zpre<L, 16, 512><<<grid, 512, 0, s>>>(a);
-    } else {
-        return hipErrorInvalidValue;
-    }
+    if (W != 8) return hipErrorInvalidValue;
+    if (a.p && a.kft) k_fft_zpre<L, 8, 256, true, true><<<grid, 256, 0, s>>>(a);
+    else if (a.p) k_fft_zpre<L, 8, 256, true, false><<<grid, 256, 0, s>>>(a);
+    else if (a.kft) k_fft_zpre<L, 8, 256, false, true><<<grid, 256, 0, s>>>(a);
+    else k_fft_zpre<L, 8, 256, false, false><<<grid, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
 template <int L>
 static hipError_t zpost_w(int W, const ZArgs& a, long long nouter, hipStream_t s) {
     const dim3 grid((unsigned)nouter);
-    if (W == 8) {
-        k_fft_zpost<L, 8, 256><<<grid, 256, 0, s>>>(a);
-    } else if (W == 16 && L * 17 <= 8192) {
-        if constexpr (L * 17 <= 8192) k_fft_zpost<L, 16, 512><<<grid, 512, 0, s>>>(a);
-    } else {
-        return hipErrorInvalidValue;
-    }
+    if (W != 8) return hipErrorInvalidValue;
+    k_fft_zpost<L, 8, 256><<<grid, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 
