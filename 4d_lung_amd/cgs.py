@@ -93,6 +93,15 @@ def lib():
         "cgs_profile_report": (I, [P, ctypes.c_char_p, I]),
         "cgs_profile_reset": (I, [P]),
         "cgs_device_bytes": (LL, [P]),
+        "cgs_set_stream": (I, [P, P]),
+        "cgs_pts_create": (I, [P, ctypes.POINTER(P)]),
+        "cgs_pts_destroy": (I, [P]),
+        "cgs_pts_set_coords": (I, [P, LL, P]),
+        "cgs_pts_count": (I, [P, ctypes.POINTER(LL)]),
+        "cgs_pts_forward": (I, [P, P, P]),
+        "cgs_pts_adjoint": (I, [P, P, P]),
+        "cgs_pts_grad_coords_fwd": (I, [P, P, P, P]),
+        "cgs_pts_grad_coords_adj": (I, [P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -55,7 +55,8 @@ struct cgs_ctx {
     Tiles tiles{};
     KbParams kb{};
     bool kb_ready = false;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;      // stream every call runs on
+    hipStream_t own_st = nullptr;  // the context's own stream (st unless cgs_set_stream chose another)
     std::string err;
     long long dev_bytes = 0;
 
@@ -116,6 +117,27 @@ struct cgs_fft {
     float2* d_a = nullptr;
     float2* d_tw[3] = {nullptr, nullptr, nullptr};
     hipStream_t st = nullptr;
+};
+
+// A free-standing set of m sample coordinates on a context's grid (the
+// differentiable operators): tile-sorted tables like a bin's, plus the KB
+// derivative table.  Valid for the tile shape and kernel it was built with.
+struct cgs_pts {
+    cgs_ctx* h = nullptr;
+    long long m = 0;          // samples
+    long long cap = 0;        // samples the coordinate buffer holds
+    double* k64 = nullptr;    // [cap][3] coordinates widened to double
+    int* perm = nullptr;      // [m]
+    int* tile_off = nullptr;  // [ntiles + 1]
+    int4* base = nullptr;     // [m]
+    double* vals = nullptr;   // [m][VS]
+    double* dvals = nullptr;  // [m][VS]
+    int* sbin = nullptr;      // [m] zeros (every sample in bin 0)
+    float2* ys = nullptr;     // [m][C] sorted sample values (scratch)
+    int tcfg = -1;
+    int kw = 0;
+    float w = 0.0f, b = 0.0f;
+    bool ready = false;
 };
 
 namespace {
@@ -315,43 +337,52 @@ void free_packed(cgs_ctx* h) {
     if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
 }
 
-// Binning + stable tile sort + KB tables of one bin.  Active samples: coil-0
-// value of d_data0 != 0 (when given), else d_mask != 0 (when given), else all.
+// Binning + stable tile sort + KB tables of the active samples among traj[ns]
+// (coil-0 value of d_data0 != 0 when given, else d_mask != 0 when given, else
+// all).  Allocates *perm, *tile_off, *base, *vals.
+int sort_samples(cgs_ctx* h, const double* traj, long long ns, const float2* d_data0, const uint8_t* d_mask,
+                 long long* m_out, int** perm, int** tile_off, int4** base, double** vals) {
+    if (!h->kb_ready) CKS(setup_kb(h));
+    const int nt = h->tiles.ntiles;
+    int *key = nullptr, *counts = nullptr, *tmp = nullptr, *cursor = nullptr;
+    CKS(dalloc(h, &key, std::max(1LL, ns)));
+    CKS(dalloc(h, &counts, nt));
+    CKS(dalloc(h, &cursor, nt));
+    CKS(dalloc(h, tile_off, nt + 1));
+    CK(hipMemsetAsync(counts, 0, sizeof(int) * nt, h->st));
+    CK(hipMemsetAsync(cursor, 0, sizeof(int) * nt, h->st));
+    if (ns > 0) CK(launch_bin_keys(traj, ns, d_data0, d_mask, geom(h), h->tiles, key, nullptr, counts, h->st));
+    CK(launch_scan_offsets(counts, nt, *tile_off, h->st));
+    int m = 0;
+    CK(hipMemcpyAsync(&m, *tile_off + nt, sizeof(int), hipMemcpyDeviceToHost, h->st));
+    CK(hipStreamSynchronize(h->st));
+    *m_out = m;
+    const int vs = val_stride(h->kb.kw);
+    CKS(dalloc(h, &tmp, std::max(1, m)));
+    CKS(dalloc(h, perm, m));
+    CKS(dalloc(h, base, m));
+    CKS(dalloc(h, vals, (size_t)m * vs));
+    if (m > 0) {
+        CK(launch_scatter(key, ns, *tile_off, cursor, tmp, h->st));
+        CK(launch_segment_rank(tmp, *tile_off, nt, *perm, h->st));
+        CK(launch_sample_tables(traj, *perm, m, geom(h), h->kb, *base, *vals, h->st));
+    }
+    CK(hipStreamSynchronize(h->st));
+    dfree(h, key, std::max(1LL, ns));
+    dfree(h, counts, nt);
+    dfree(h, cursor, nt);
+    dfree(h, tmp, std::max(1, m));
+    return CGS_OK;
+}
+
+// Binning + stable tile sort + KB tables of one bin.
 int build_bin(cgs_ctx* h, int bin, const float2* d_data0, const uint8_t* d_mask) {
     if (!h->d_traj) return fail(h, CGS_ERR_STATE, "no trajectory set");
     if (!h->kb_ready) CKS(setup_kb(h));
     free_packed(h);
     BinData& B = h->bins[bin];
     free_bin(h, B);
-    const int nt = h->tiles.ntiles;
-    int *key = nullptr, *counts = nullptr, *tmp = nullptr, *cursor = nullptr;
-    CKS(dalloc(h, &key, h->ns));
-    CKS(dalloc(h, &counts, nt));
-    CKS(dalloc(h, &cursor, nt));
-    CKS(dalloc(h, &B.tile_off, nt + 1));
-    CK(hipMemsetAsync(counts, 0, sizeof(int) * nt, h->st));
-    CK(hipMemsetAsync(cursor, 0, sizeof(int) * nt, h->st));
-    CK(launch_bin_keys(h->d_traj, h->ns, d_data0, d_mask, geom(h), h->tiles, key, nullptr, counts, h->st));
-    CK(launch_scan_offsets(counts, nt, B.tile_off, h->st));
-    int m = 0;
-    CK(hipMemcpyAsync(&m, B.tile_off + nt, sizeof(int), hipMemcpyDeviceToHost, h->st));
-    CK(hipStreamSynchronize(h->st));
-    B.m = m;
-    const int vs = val_stride(h->kb.kw);
-    CKS(dalloc(h, &tmp, std::max(1, m)));
-    CKS(dalloc(h, &B.perm, m));
-    CKS(dalloc(h, &B.base, m));
-    CKS(dalloc(h, &B.vals, (size_t)m * vs));
-    if (m > 0) {
-        CK(launch_scatter(key, h->ns, B.tile_off, cursor, tmp, h->st));
-        CK(launch_segment_rank(tmp, B.tile_off, nt, B.perm, h->st));
-        CK(launch_sample_tables(h->d_traj, B.perm, m, geom(h), h->kb, B.base, B.vals, h->st));
-    }
-    CK(hipStreamSynchronize(h->st));
-    dfree(h, key, h->ns);
-    dfree(h, counts, nt);
-    dfree(h, cursor, nt);
-    dfree(h, tmp, std::max(1, m));
+    CKS(sort_samples(h, h->d_traj, h->ns, d_data0, d_mask, &B.m, &B.perm, &B.tile_off, &B.base, &B.vals));
     B.built = true;
     return CGS_OK;
 }
@@ -657,10 +688,11 @@ int cgs_create(const int grid[3], const int samp[3], int ncoil, cgs_ctx** out) {
     h->ns = (long long)samp[0] * samp[1] * samp[2];
     h->C = ncoil;
     h->bins.resize(1);
-    if (hipStreamCreate(&h->st) != hipSuccess) {
+    if (hipStreamCreate(&h->own_st) != hipSuccess) {
         delete h;
         return CGS_ERR_HIP;
     }
+    h->st = h->own_st;
     if (upload_twiddles(h, h->N, h->d_tw) != CGS_OK) {
         cgs_destroy(h);
         return CGS_ERR_HIP;
@@ -688,7 +720,7 @@ int cgs_destroy(cgs_ctx* h) {
     dfree(h, h->d_x, n); dfree(h, h->d_r, n); dfree(h, h->d_p, n); dfree(h, h->d_q, n);
     dfree(h, h->d_part, 1); dfree(h, h->d_part2, 1); dfree(h, h->d_sc, 1); dfree(h, h->d_done, 1);
     dfree(h, h->d_hist, 1);
-    if (h->st) (void)hipStreamDestroy(h->st);
+    if (h->own_st) (void)hipStreamDestroy(h->own_st);
     delete h;
     return CGS_OK;
 }
@@ -1320,6 +1352,183 @@ int cgs_set_bin_voronoi(cgs_ctx* h, int bin) {
     if (!B.dcf_full) CKS(dalloc(h, &B.dcf_full, h->ns));
     CK(hipMemcpy(B.dcf_full, full.data(), sizeof(float) * h->ns, hipMemcpyHostToDevice));
     free_packed(h);
+    return CGS_OK;
+}
+
+// ------------------------------------------------------ differentiable ops
+int cgs_set_stream(cgs_ctx* h, void* stream) {
+    if (!h) return CGS_ERR_ARG;
+    CK(hipStreamSynchronize(h->st));
+    h->st = stream ? (hipStream_t)stream : h->own_st;
+    return CGS_OK;
+}
+
+namespace {
+
+void pts_free_tables(cgs_pts* p) {
+    cgs_ctx* h = p->h;
+    const int vs = val_stride(p->kw > 0 ? p->kw : h->kb.kw);
+    dfree(h, p->perm, p->m);
+    dfree(h, p->tile_off, h->tiles.ntiles + 1);
+    dfree(h, p->base, p->m);
+    dfree(h, p->vals, p->m * vs);
+    dfree(h, p->dvals, p->m * vs);
+    dfree(h, p->sbin, std::max(1LL, p->m));
+    dfree(h, p->ys, p->m * h->C);
+    p->m = 0;
+    p->ready = false;
+}
+
+int pts_check(cgs_pts* p, bool need_maps) {
+    if (!p || !p->h) return CGS_ERR_ARG;
+    cgs_ctx* h = p->h;
+    if (!p->ready) return fail(h, CGS_ERR_STATE, "point set has no coordinates");
+    if (p->tcfg != h->tcfg || p->kw != h->kb.kw || p->w != h->w || p->b != h->b || !h->kb_ready)
+        return fail(h, CGS_ERR_STATE, "point set was built for another tile shape or kernel; set its coordinates again");
+    if (need_maps && !h->maps_ready) return fail(h, CGS_ERR_STATE, "sensitivity maps not set");
+    if (!h->kft_ready) return fail(h, CGS_ERR_STATE, "kernelFT not computed");
+    CKS(ensure_grid(h, 1));
+    return CGS_OK;
+}
+
+GridArgs pts_grid_args(cgs_pts* p) {
+    cgs_ctx* h = p->h;
+    GridArgs g{};
+    g.G = h->d_grid;
+    g.Y = p->ys;
+    g.s.base = p->base;
+    g.s.vals = p->vals;
+    g.s.dcf = nullptr;
+    g.s.sbin = p->sbin;
+    g.s.first = 0;
+    g.s.m = p->m;
+    g.g = geom(h);
+    g.t = h->tiles;
+    g.tile_off = p->tile_off;
+    g.gdcf = nullptr;
+    g.thr = h->kb.thr;
+    g.b0 = 0;
+    g.nb = 1;
+    g.done = nullptr;
+    return g;
+}
+
+// Coordinate gradient of <B, degrid(grid)> on the context's grid buffer.
+int pts_grad(cgs_pts* p, const float2* b_caller, float scale, float* gk) {
+    cgs_ctx* h = p->h;
+    CK(launch_permute_rows(b_caller, p->base, p->m, h->C, 0, p->ys, h->st));
+    GradKArgs a{};
+    a.A = h->d_grid;
+    a.B = p->ys;
+    a.base = p->base;
+    a.vals = p->vals;
+    a.dvals = p->dvals;
+    a.m = p->m;
+    a.g = geom(h);
+    a.thr = h->kb.thr;
+    a.scale = scale;
+    a.gk = gk;
+    {
+        Prof t(h, "grad_k");
+        CK(launch_grad_k(h->kb.kw, a, h->st));
+    }
+    return CGS_OK;
+}
+
+}  // namespace
+
+int cgs_pts_create(cgs_ctx* h, cgs_pts** out) {
+    if (!h || !out) return CGS_ERR_ARG;
+    cgs_pts* p = new cgs_pts;
+    p->h = h;
+    *out = p;
+    return CGS_OK;
+}
+
+int cgs_pts_destroy(cgs_pts* p) {
+    if (!p) return CGS_OK;
+    cgs_ctx* h = p->h;
+    (void)hipStreamSynchronize(h->st);
+    pts_free_tables(p);
+    dfree(h, p->k64, (size_t)p->cap * 3);
+    delete p;
+    return CGS_OK;
+}
+
+int cgs_pts_set_coords(cgs_pts* p, long long m, const float* k_dev) {
+    if (!p || !p->h || m < 0 || (m > 0 && !k_dev)) return CGS_ERR_ARG;
+    cgs_ctx* h = p->h;
+    if (m > (long long)INT32_MAX) return fail(h, CGS_ERR_ARG, "too many samples");
+    if (!h->kb_ready) CKS(setup_kb(h));
+    pts_free_tables(p);
+    if (m > p->cap) {
+        dfree(h, p->k64, (size_t)p->cap * 3);
+        p->cap = 0;
+        CKS(dalloc(h, &p->k64, (size_t)m * 3));
+        p->cap = m;
+    }
+    CK(launch_widen_coords(k_dev, m * 3, p->k64, h->st));
+    long long mm = 0;
+    CKS(sort_samples(h, p->k64, m, nullptr, nullptr, &mm, &p->perm, &p->tile_off, &p->base, &p->vals));
+    p->m = mm;   // every point is active: mm == m
+    const int vs = val_stride(h->kb.kw);
+    CKS(dalloc(h, &p->dvals, (size_t)mm * vs));
+    CKS(dalloc(h, &p->sbin, std::max(1LL, mm)));
+    CKS(dalloc(h, &p->ys, (size_t)mm * h->C));
+    CK(hipMemsetAsync(p->sbin, 0, sizeof(int) * std::max(1LL, mm), h->st));
+    CK(launch_sample_dtables(p->k64, p->perm, mm, geom(h), h->kb, p->dvals, h->st));
+    p->tcfg = h->tcfg;
+    p->kw = h->kb.kw;
+    p->w = h->w;
+    p->b = h->b;
+    p->ready = true;
+    return CGS_OK;
+}
+
+int cgs_pts_forward(cgs_pts* p, const float* x_dev, float* y_dev) {
+    CKS(pts_check(p, true));
+    cgs_ctx* h = p->h;
+    CKS(run_fwd_fft(h, 0, 1, (const float2*)x_dev, true, nullptr));
+    GridArgs g = pts_grid_args(p);
+    {
+        Prof t(h, "degrid");
+        CK(launch_degrid(h->kb.kw, g, h->st));
+    }
+    CK(launch_permute_rows(p->ys, p->base, p->m, h->C, 1, (float2*)y_dev, h->st));
+    return CGS_OK;
+}
+
+int cgs_pts_adjoint(cgs_pts* p, const float* y_dev, float* x_dev) {
+    CKS(pts_check(p, true));
+    cgs_ctx* h = p->h;
+    CK(launch_permute_rows((const float2*)y_dev, p->base, p->m, h->C, 0, p->ys, h->st));
+    GridArgs g = pts_grid_args(p);
+    {
+        Prof t(h, "grid");
+        CK(launch_grid(h->kb.kw, h->tcfg, g, h->st));
+    }
+    CKS(run_inv_fft(h, 1, (float2*)x_dev, nullptr, nullptr, nullptr));
+    return CGS_OK;
+}
+
+int cgs_pts_grad_coords_fwd(cgs_pts* p, const float* x_dev, const float* gy_dev, float* gk_dev) {
+    CKS(pts_check(p, true));
+    cgs_ctx* h = p->h;
+    CKS(run_fwd_fft(h, 0, 1, (const float2*)x_dev, true, nullptr));
+    return pts_grad(p, (const float2*)gy_dev, 1.0f, gk_dev);
+}
+
+int cgs_pts_grad_coords_adj(cgs_pts* p, const float* y_dev, const float* gx_dev, float* gk_dev) {
+    CKS(pts_check(p, true));
+    cgs_ctx* h = p->h;
+    CKS(run_fwd_fft(h, 0, 1, (const float2*)gx_dev, true, nullptr));
+    const float s = (float)(1e-3 / std::sqrt((double)h->ng));
+    return pts_grad(p, (const float2*)y_dev, s, gk_dev);
+}
+
+int cgs_pts_count(const cgs_pts* p, long long* m) {
+    if (!p || !m) return CGS_ERR_ARG;
+    *m = p->m;
     return CGS_OK;
 }
 

@@ -71,6 +71,11 @@ hipError_t launch_segment_rank(const int* tmp, const int* off, int ntiles, int* 
 hipError_t launch_sample_tables(const double* traj, const int* perm, long long m, Geom g, KbParams kb,
                                 int4* base, double* vals, hipStream_t s);
 hipError_t launch_kernel_ft(const double* axr, const double* axi, Geom g, float* kft, hipStream_t s);
+hipError_t launch_sample_dtables(const double* traj, const int* perm, long long m, Geom g, KbParams kb,
+                                 double* dvals, hipStream_t s);
+hipError_t launch_widen_coords(const float* k32, long long n, double* k64, hipStream_t s);
+hipError_t launch_permute_rows(const float2* in, const int4* base, long long m, int C, int to_caller, float2* out,
+                               hipStream_t s);
 hipError_t launch_gather_data(const float2* data, long long ns, const int4* base, long long m, int C,
                               const float* dcf_full, const int* perm, float* dcf_sorted, float2* y, hipStream_t s);
 hipError_t launch_scatter_data(const float2* y, const int4* base, long long m, int C, long long ns,
@@ -105,6 +110,25 @@ struct GridArgs {
 
 hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s);
 hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s);
+
+// Coordinate gradient of the sample values: for every sample i (sorted
+// position s, original index base[s].w)
+//   gk[i][axis] = scale * sum_c sum_taps Re(conj(B[s][c]) * A[tap][c]) * dW_tap/dk_axis
+// over the kept taps (W > thr), dW/dk from the per-axis value and derivative
+// tables.  Serves both d<gy, degrid(A)>/dk (B = gy) and d<A, grid(B)>/dk.
+struct GradKArgs {
+    const float2* A;       // [Ng][C] grid
+    const float2* B;       // [m][C] sample values, sorted order
+    const int4* base;
+    const double* vals;
+    const double* dvals;
+    long long m;
+    Geom g;
+    double thr;
+    float scale;
+    float* gk;             // [m][3], original order
+};
+hipError_t launch_grad_k(int kw, const GradKArgs& a, hipStream_t s);
 int grid_tile_configs();
 void grid_tile_dims(int tcfg, int* tx, int* ty, int* tz);
 

@@ -245,6 +245,66 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     }
 }
 
+// ------------------------------------------------------ coordinate gradient
+// Lane per (sample, coil) as in k_degrid; the three partial sums of a sample
+// are reduced over its CW lanes with shuffles.  Taps are the forward's kept
+// taps (float((vx*vy)*vz) > thr); the derivative of each is
+// (dvx*vy*vz, vx*dvy*vz, vx*vy*dvz) in FP64, accumulated in FP32.
+template <int KW, int CW>
+__global__ __launch_bounds__(CGS_THREADS) void k_grad_k(GradKArgs a) {
+    constexpr int VS = val_stride(KW);
+    const long long e = (long long)blockIdx.x * CGS_THREADS + threadIdx.x;
+    const long long s = e / CW;
+    const int lane_c = (int)(e % CW);
+    const bool valid = s < a.m;
+    float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+    if (valid) {
+        const int4 bs = a.base[s];
+        const double* __restrict__ v = a.vals + s * VS;
+        const double* __restrict__ dv = a.dvals + s * VS;
+        const int C = a.g.C;
+        for (int c = lane_c; c < C; c += CW) {
+            const float2 bv = a.B[s * C + c];
+            for (int ox = 0; ox < KW; ++ox) {
+                const int xg = bs.x + ox;
+                if (xg < 0 || xg >= a.g.nx) continue;
+                for (int oy = 0; oy < KW; ++oy) {
+                    const int yg = bs.y + oy;
+                    if (yg < 0 || yg >= a.g.ny) continue;
+                    const double vxy = v[ox] * v[KW + oy];
+                    const double dxy = dv[ox] * v[KW + oy];
+                    const double xdy = v[ox] * dv[KW + oy];
+                    const long long row = ((long long)xg * a.g.ny + yg) * a.g.nz;
+#pragma unroll
+                    for (int oz = 0; oz < KW; ++oz) {
+                        const int zg = bs.z + oz;
+                        if (zg < 0 || zg >= a.g.nz) continue;
+                        const float w = (float)(vxy * v[2 * KW + oz]);
+                        if (!((double)w > a.thr)) continue;
+                        const float2 av = a.A[(row + zg) * C + c];
+                        const float t = bv.x * av.x + bv.y * av.y;
+                        g0 = fmaf(t, (float)(dxy * v[2 * KW + oz]), g0);
+                        g1 = fmaf(t, (float)(xdy * v[2 * KW + oz]), g1);
+                        g2 = fmaf(t, (float)(vxy * dv[2 * KW + oz]), g2);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = CW / 2; off >= 1; off >>= 1) {
+        g0 += __shfl_xor(g0, off, 64);
+        g1 += __shfl_xor(g1, off, 64);
+        g2 += __shfl_xor(g2, off, 64);
+    }
+    if (valid && lane_c == 0) {
+        const long long i = a.base[s].w;
+        a.gk[3 * i + 0] = a.scale * g0;
+        a.gk[3 * i + 1] = a.scale * g1;
+        a.gk[3 * i + 2] = a.scale * g2;
+    }
+}
+
 // -------------------------------------------------------------- dispatch
 hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s) {
     const int C = a.g.C;
@@ -316,6 +376,32 @@ hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s) {
         case 7: return grid_tile<7>(tcfg, a, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_grad_k(int kw, const GradKArgs& a, hipStream_t s) {
+    if (a.m == 0) return hipSuccess;
+    const int cw = pow2_at_least(a.g.C, 64);
+    const dim3 grid((unsigned)((a.m * cw + CGS_THREADS - 1) / CGS_THREADS));
+#define CGS_GK(KWv, CWv) k_grad_k<KWv, CWv><<<grid, CGS_THREADS, 0, s>>>(a)
+#define CGS_GK_CW(KWv)                         \
+    switch (cw) {                              \
+        case 1: CGS_GK(KWv, 1); break;         \
+        case 2: CGS_GK(KWv, 2); break;         \
+        case 4: CGS_GK(KWv, 4); break;         \
+        case 8: CGS_GK(KWv, 8); break;         \
+        case 16: CGS_GK(KWv, 16); break;       \
+        case 32: CGS_GK(KWv, 32); break;       \
+        default: CGS_GK(KWv, 64); break;       \
+    }
+    switch (kw) {
+        case 3: CGS_GK_CW(3); break;
+        case 5: CGS_GK_CW(5); break;
+        case 7: CGS_GK_CW(7); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef CGS_GK_CW
+#undef CGS_GK
+    return hipGetLastError();
 }
 
 }  // namespace cgs

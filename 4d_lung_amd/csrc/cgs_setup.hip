@@ -157,6 +157,77 @@ __global__ void k_sample_tables(const double* __restrict__ traj, const int* __re
     base[s] = make_int4(b3[0], b3[1], b3[2], i);
 }
 
+// I1(x) / x by its power series (Abramowitz & Stegun 9.6.10 divided by x):
+// (1/2) sum_k (x^2/4)^k / (k! (k+1)!), finite at x = 0.
+__device__ double i1_over_x(double x) {
+    const double q = (0.25 * x) * x;
+    double term = 0.5;
+    double sum = 0.5;
+    for (int k = 1; k < 500; ++k) {
+        term = (term * q) / ((double)k * (double)(k + 1));
+        sum = sum + term;
+        if (term < sum * 1.0e-17) break;
+    }
+    return sum;
+}
+
+// d/dk of one axis factor of convFunction (Reco.h:228-240) at offset
+// e = k - idx*dk, taken analytically: with q = 2|e|/(w dk) and
+// beta = b sqrt(1 - q^2), f = I0(beta)/denom and
+//   df/dk = -sign(e) (2 b^2 q / (w dk)) (I1(beta)/beta) / denom,
+// which stays finite at the window edge (beta -> 0).  0 beyond the cut-off,
+// like the factor itself.
+__device__ double kb_axis_dk(double e, const AxisConst& c, double b) {
+    const float dxf = (float)fabs(e);
+    if (dxf > c.cut) return 0.0;
+    const double q = (double)((2.0f * dxf) / c.wdk);
+    const double beta = b * sqrt(1.0 - q * q);
+    const double g = -((2.0 * b * b * q) / (double)c.wdk) * i1_over_x(beta) / (double)c.denom;
+    return e < 0.0 ? -g : g;
+}
+
+// Per-axis KB derivative table of the sorted samples, laid out like vals.
+__global__ void k_sample_dtables(const double* __restrict__ traj, const int* __restrict__ perm, long long m,
+                                 Geom g, KbParams kb, double* __restrict__ dvals) {
+    const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= m) return;
+    const int i = perm[s];
+    const int n3[3] = {g.nx, g.ny, g.nz};
+    const int vs = val_stride(kb.kw);
+    for (int a = 0; a < 3; ++a) {
+        const double k = traj[3LL * i + a];
+        const int b0 = nearest_pt(n3[a], k) - kb.h;
+        for (int o = 0; o < kb.kw; ++o) {
+            const int idx = b0 + o;
+            double v = 0.0;
+            if (idx >= 0 && idx <= n3[a] - 1) {
+                const double e = k - (double)((float)idx * kb.ax[a].dkf);
+                v = kb_axis_dk(e, kb.ax[a], kb.b);
+            }
+            dvals[s * vs + a * kb.kw + o] = v;
+        }
+    }
+    for (int o = 3 * kb.kw; o < vs; ++o) dvals[s * vs + o] = 0.0;
+}
+
+__global__ void k_widen_coords(const float* __restrict__ k32, long long n, double* __restrict__ k64) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) k64[i] = (double)k32[i];
+}
+
+// Rows of C complex values between caller order (row = original index) and
+// sorted order (row = sorted position s, original index base[s].w).
+__global__ void k_permute_rows(const float2* __restrict__ in, const int4* __restrict__ base, long long m, int C,
+                               int to_caller, float2* __restrict__ out) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m * C) return;
+    const long long s = e / C;
+    const int c = (int)(e % C);
+    const long long i = base[s].w;
+    if (to_caller) out[i * C + c] = in[e];
+    else out[e] = in[i * C + c];
+}
+
 // kernelFT = 1 / Re(Xx Xy Xz) with the per-axis centred inverse DFTs of the
 // sampled kernel (Reco.cpp:560-601), rounded to float.
 __global__ void k_kernel_ft(const double* __restrict__ axr, const double* __restrict__ axi, Geom g,
@@ -248,6 +319,26 @@ hipError_t launch_sample_tables(const double* traj, const int* perm, long long m
                                 double* vals, hipStream_t s) {
     if (m == 0) return hipSuccess;
     k_sample_tables<<<nblocks(m, 128), 128, 0, s>>>(traj, perm, m, g, kb, base, vals);
+    return hipGetLastError();
+}
+
+hipError_t launch_sample_dtables(const double* traj, const int* perm, long long m, Geom g, KbParams kb,
+                                 double* dvals, hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    k_sample_dtables<<<nblocks(m, 128), 128, 0, s>>>(traj, perm, m, g, kb, dvals);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen_coords(const float* k32, long long n, double* k64, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_widen_coords<<<nblocks(n, 256), 256, 0, s>>>(k32, n, k64);
+    return hipGetLastError();
+}
+
+hipError_t launch_permute_rows(const float2* in, const int4* base, long long m, int C, int to_caller, float2* out,
+                               hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    k_permute_rows<<<nblocks(m * C, 256), 256, 0, s>>>(in, base, m, C, to_caller, out);
     return hipGetLastError();
 }
 

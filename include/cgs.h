@@ -137,6 +137,40 @@ int cgs_voronoi_density(const double* pts, long long np, int average, int sx, do
  * weights (per-bin DCF of the 4D workloads). */
 int cgs_set_bin_voronoi(cgs_ctx* h, int bin);
 
+/* ---- differentiable operators on device memory ----
+ * For a binding that differentiates through E and E^H (unrolled CG).  All
+ * pointers below are device pointers; calls are ordered on the context's
+ * stream (see cgs_set_stream) and return without synchronising.  Layouts:
+ *   x     image / CG vector, Ng complex ([x][y][z])
+ *   y     samples [m][C] complex (coil fastest), in the point set's order
+ *   k     coordinates [m][3] float32 in [0,1] per axis (as Trajectory.h:32)
+ *   gk    [m][3] float32
+ * E and E^H are exactly Reco::nufftFWD / nufftBWD with the context's maps,
+ * kernel and deapodization (Reco.cpp:468-553), so <E x, y> * 1e-3/sqrt(Ng)
+ * = <x, E^H y>.  No counterpart in the reference, which has no gradients. */
+typedef struct cgs_pts cgs_pts;
+/* Run every later call of the context on `stream` (a hipStream_t of the
+ * caller, default priority; NULL = the context's own stream). */
+int cgs_set_stream(cgs_ctx* h, void* stream);
+/* A point set on the context's grid; destroy it before the context. */
+int cgs_pts_create(cgs_ctx* h, cgs_pts** out);
+int cgs_pts_destroy(cgs_pts* p);
+/* Coordinates of m samples: binning, tile sort, KB value and KB derivative
+ * tables (the sample-to-grid rules of ConvKernel::calculateWeights,
+ * Reco.cpp:698-787).  Every sample is active. */
+int cgs_pts_set_coords(cgs_pts* p, long long m, const float* k_dev);
+int cgs_pts_count(const cgs_pts* p, long long* m);
+/* y = E x (Reco::nufftFWD of the coil images x * S_c). */
+int cgs_pts_forward(cgs_pts* p, const float* x_dev, float* y_dev);
+/* x = E^H y (Reco::nufftBWD + conj(S_c) coil sum). */
+int cgs_pts_adjoint(cgs_pts* p, const float* y_dev, float* x_dev);
+/* gk[i] = d/dk_i Re<gy, E x> = sum_c sum_taps Re(conj(gy[i][c]) G_c[tap]) dW/dk_i,
+ * G_c = FFT(kernelFT * S_c * x), with the analytic derivative of the KB
+ * kernel (Reco.h:218-242) over the kept taps. */
+int cgs_pts_grad_coords_fwd(cgs_pts* p, const float* x_dev, const float* gy_dev, float* gk_dev);
+/* gk[i] = d/dk_i Re<gx, E^H y>. */
+int cgs_pts_grad_coords_adj(cgs_pts* p, const float* y_dev, const float* gx_dev, float* gk_dev);
+
 /* ---- Fourier (Fourier.h:12-54) on one complex grid ---- */
 int cgs_fft_create(const int dims[3], cgs_fft** out);
 int cgs_fft_destroy(cgs_fft* f);

@@ -291,6 +291,11 @@ struct Oracle {
     std::vector<double> traj;                      // nS x 3, memory order
     std::vector<std::vector<int>> idx;             // ConvKernel::indices (Reco.h:59)
     std::vector<std::vector<float>> wts;           // ConvKernel::weights (Reco.h:58)
+    // Smooth-kernel mode (finite-difference checks of coordinate gradients
+    // only): weights evaluated without the float roundings of the distance
+    // and of q, kept in FP64 and used by the prec-64 operators.
+    bool exact_w = false;
+    std::vector<std::vector<double>> wts64;
     std::vector<float> kft;                        // ConvKernel::kernelFT real part (Reco.h:60)
     std::vector<float> dcf;                        // Reco::denCorr (Reco.h:86)
     std::vector<Cplx<float>> maps;                 // Reco::sMaps (Reco.h:84), C x Ng
@@ -307,6 +312,19 @@ void collect_neighbor_indices(const Oracle& o, const int* start, std::vector<int
     }
 }
 
+// The per-axis factor of kb_axis_value as a smooth function of a double
+// distance: the reference's constants (dkf, float w and b) but no float
+// rounding of the distance or of q, so that finite differences in the sample
+// coordinates see the kernel's derivative rather than rounding steps.  Not a
+// statement of the reference; used only in smooth-kernel mode.
+double kb_axis_exact(double d, int N, float w, float b) {
+    const double dk = (double)(float)(1.0 / (double)(float)(N - 1));
+    const double W = (double)w * dk;
+    if (d > 0.5 * W) return 0.0;
+    const double q = 2.0 * d / W;
+    return i0_series((double)b * std::sqrt(1.0 - q * q)) / (100.0 * (double)w * dk);
+}
+
 // Threshold of Reco.cpp:716-717: convFunction({0,0,0}) * 1e-7 (float promoted).
 double kernel_threshold(const Oracle& o) {
     const double zero[3] = {0.0, 0.0, 0.0};
@@ -319,6 +337,7 @@ double kernel_threshold(const Oracle& o) {
 void calculate_weights(Oracle& o, const uint8_t* mask) {
     o.idx.assign(o.nS, {});
     o.wts.assign(o.nS, {});
+    o.wts64.assign(o.exact_w ? o.nS : 0, {});
     const float dkx = (float)(1.0 / (double)(float)(o.N[0] - 1));
     const float dky = (float)(1.0 / (double)(float)(o.N[1] - 1));
     const float dkz = (float)(1.0 / (double)(float)(o.N[2] - 1));
@@ -349,6 +368,16 @@ void calculate_weights(Oracle& o, const uint8_t* mask) {
             const double ey = k[1] - (double)((float)ind[1] * dky);
             const double ez = k[2] - (double)((float)ind[2] * dkz);
             const double dist[3] = {std::sqrt(ex * ex), std::sqrt(ey * ey), std::sqrt(ez * ez)};
+            if (o.exact_w) {
+                const double kv = (kb_axis_exact(dist[0], o.N[0], o.w, o.b) * kb_axis_exact(dist[1], o.N[1], o.w, o.b)) *
+                                  kb_axis_exact(dist[2], o.N[2], o.w, o.b);
+                if (kv > thr) {
+                    o.idx[i].push_back(pts[j]);
+                    o.wts[i].push_back((float)kv);
+                    o.wts64[i].push_back(kv);
+                }
+                continue;
+            }
             const double kv = (double)conv_function(dist, o.N, o.w, o.b);
             if (kv > thr) {
                 o.idx[i].push_back(pts[j]);
@@ -435,8 +464,9 @@ void nufft_bwd_coil(const Oracle& o, const Cplx<T>* data, Cplx<T>* img) {
         const T d0 = data[i].re, d1 = data[i].im;
         const auto& ix = o.idx[i];
         const auto& wv = o.wts[i];
+        const bool ex = o.exact_w && sizeof(T) == sizeof(double);
         for (size_t t = 0; t < ix.size(); t++) {
-            const T kv = (T)wv[t];
+            const T kv = ex ? (T)o.wts64[i][t] : (T)wv[t];
             img[ix[t]].re += d0 * kv;
             img[ix[t]].im += d1 * kv;
         }
@@ -467,8 +497,9 @@ void nufft_fwd_coil(const Oracle& o, Cplx<T>* img, Cplx<T>* data) {
         T a0 = 0, a1 = 0;
         const auto& ix = o.idx[i];
         const auto& wv = o.wts[i];
+        const bool ex = o.exact_w && sizeof(T) == sizeof(double);
         for (size_t t = 0; t < ix.size(); t++) {
-            const double kv = wv[t];
+            const double kv = ex ? o.wts64[i][t] : (double)wv[t];
             a0 = (T)((double)a0 + (double)img[ix[t]].re * kv);
             a1 = (T)((double)a1 + (double)img[ix[t]].im * kv);
         }
@@ -581,8 +612,12 @@ void orc_exclude_missing_data(void* h, const float* data) {
     for (long i = 0; i < o->nS; i++)
         if (data[2 * i] == 0.0f && data[2 * i + 1] == 0.0f) {
             o->idx[i].clear(); o->wts[i].clear();
+            if (o->exact_w) o->wts64[i].clear();
         }
 }
+
+// Smooth-kernel mode for finite-difference checks (set before the weights).
+void orc_set_exact_weights(void* h, int on) { ((Oracle*)h)->exact_w = on != 0; }
 
 long orc_num_taps(void* h) {
     Oracle* o = (Oracle*)h;

@@ -45,6 +45,7 @@ def lib():
         L.orc_set_img_range.argtypes = [P, I3, I3]
         L.orc_set_trajectory.argtypes = [P, P]
         L.orc_calculate_weights.argtypes = [P, P]
+        L.orc_set_exact_weights.argtypes = [P, ctypes.c_int]
         L.orc_exclude_missing_data.argtypes = [P, P]
         L.orc_num_taps.restype = ctypes.c_long
         L.orc_num_taps.argtypes = [P]
@@ -168,6 +169,11 @@ class OracleReco:
         traj = np.ascontiguousarray(traj, dtype=np.float64)
         assert traj.shape == (self.ns, 3)
         lib().orc_set_trajectory(self.h, _ptr(traj))
+
+    def set_exact_weights(self, on=True):
+        """Smooth-kernel mode: prec-64 operators use FP64 weights without the
+        reference's float roundings (finite-difference gradient checks)."""
+        lib().orc_set_exact_weights(self.h, int(bool(on)))
 
     def calculate_weights(self, data=None):
         if data is None:
