@@ -1273,25 +1273,42 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     for (long long i = 0; i < np; i++) idx[cur[key[i]]++] = (int)i;
 
     double *dp = nullptr, *dv = nullptr;
-    int *ds = nullptr, *di = nullptr, *dov = nullptr;
+    int *ds = nullptr, *di = nullptr, *dov = nullptr, *redo = nullptr;
     CKS(dalloc(h, &dp, (size_t)np * 3));
     CKS(dalloc(h, &dv, (size_t)np));
     CKS(dalloc(h, &ds, (size_t)nc + 1));
     CKS(dalloc(h, &di, (size_t)np));
-    CKS(dalloc(h, &dov, 1));
+    CKS(dalloc(h, &dov, 2));          // [0] cells to rebuild, [1] final overflows
+    CKS(dalloc(h, &redo, (size_t)np));
     CK(hipMemcpy(dp, pts, sizeof(double) * 3 * np, hipMemcpyHostToDevice));
     CK(hipMemcpy(ds, start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
     CK(hipMemcpy(di, idx.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-    CK(hipMemset(dov, 0, sizeof(int)));
+    CK(hipMemset(dov, 0, 2 * sizeof(int)));
     H.start = ds;
     H.idx = di;
-    CK(launch_voronoi(dp, np, H, dv, dov, st));
-    int over = 0;
+    CK(launch_voronoi(dp, np, H, dv, redo, dov, st));
+    int cnt[2] = {0, 0};
+    CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    CK(hipStreamSynchronize(st));
+    if (cnt[0] > 0) {
+        // second pass: at most 2048 threads, each with a large cell workspace
+        size_t cell_b = 0, scr_b = 0;
+        voronoi_large_sizes(&cell_b, &scr_b);
+        const int nthreads = (int)std::min<long long>(((long long)cnt[0] + 63) / 64 * 64, 2048);
+        char *cells = nullptr, *scr = nullptr;
+        CKS(dalloc(h, &cells, cell_b * nthreads));
+        CKS(dalloc(h, &scr, scr_b * nthreads));
+        CK(launch_voronoi_large(dp, H, redo, cnt[0], nthreads, cells, scr, dv, dov + 1, st));
+        CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        dfree(h, cells, cell_b * nthreads);
+        dfree(h, scr, scr_b * nthreads);
+    }
+    const int over = cnt[1];
     CK(hipMemcpyAsync(vol_host, dv, sizeof(double) * np, hipMemcpyDeviceToHost, st));
-    CK(hipMemcpyAsync(&over, dov, sizeof(int), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     dfree(h, dp, (size_t)np * 3); dfree(h, dv, (size_t)np); dfree(h, ds, (size_t)nc + 1);
-    dfree(h, di, (size_t)np); dfree(h, dov, 1);
+    dfree(h, di, (size_t)np); dfree(h, dov, 2); dfree(h, redo, (size_t)np);
     if (over) return fail(h, CGS_ERR_ARG, std::to_string(over) + " Voronoi cells exceeded the face/vertex limits");
     return CGS_OK;
 }

@@ -174,7 +174,15 @@ struct VorHash {
     const int* idx;
 };
 
-hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* overflow,
+// First pass (small cell limits): volumes of all points; points whose cell
+// outgrew the limits are appended to redo[*nredo].
+hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* redo, int* nredo,
                           hipStream_t s);
+// Second pass (large limits) over redo[0..nredo) with nthreads threads (a
+// multiple of 64), each using one cell and one scratch slot of the
+// workspaces; closed cells that still overflow are counted in *overflow.
+void voronoi_large_sizes(size_t* cell, size_t* scratch);
+hipError_t launch_voronoi_large(const double* pts, const VorHash& H, const int* redo, int nredo, int nthreads,
+                                void* cells, void* scratch, double* vol, int* overflow, hipStream_t s);
 
 }  // namespace cgs

@@ -184,8 +184,11 @@ def respiratory_bins(line_mask, nbins, frac_range, seed, max_tries=1000):
     trace = np.sin(2 * np.pi * 0.25 * t) + 0.15 * rng.standard_normal(order.size)
     sorted_tr = np.sort(trace)
     lo, hi = frac_range
+    # Dirichlet concentration: 8 for up to 8 bins; for more bins the fractions
+    # must cluster tighter around 1/nbins to fit a narrow range
+    conc = 8.0 if nbins <= 8 else 40.0
     for _ in range(max_tries):
-        fr = rng.dirichlet(np.full(nbins, 8.0))
+        fr = rng.dirichlet(np.full(nbins, conc))
         if fr.min() < lo or fr.max() > hi:
             continue
         cuts = np.cumsum(fr)[:-1]

@@ -18,9 +18,12 @@
 // volume_of_finite_dual_cell (functions.h:148-150); this also zeroes the
 // cells of interior points whose Voronoi region reaches more than one hash
 // cell beyond the point cloud (near-degenerate boundary cells whose CGAL
-// volume exceeds the whole k-space).  Volume = sum over faces of area * |q - p| / 6
-// (functions.h:156-158), face areas as triangle fans from the first vertex
-// (functions.h:117-124).
+// volume exceeds the whole k-space).  Volume = sum over faces of area *
+// |q - p| / 6 (functions.h:156-158), face areas as triangle fans from the
+// first vertex (functions.h:117-124).  Cells are built with small fixed
+// limits (faces, vertices per face, distinct vertices) and, when they outgrow
+// them, again with large ones; a cell's bits do not depend on the limits it
+// fits in.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -29,10 +32,22 @@
 
 namespace vor {
 
-constexpr int FMAX = 48;   // faces of a cell
-constexpr int VMAX = 32;   // vertices of one face
-constexpr int PMAX = 96;   // distinct vertices of a cell (a simple polyhedron with F faces has <= 2F - 4)
 constexpr int RMAX = 10;   // hash shells visited at most
+
+// Fixed capacities of a cell: faces, vertices of one face, distinct vertices
+// (a simple polyhedron with F faces has at most 2F - 4), and the index type of
+// the vertex pool.  Every cell is built with the small limits first; a cell
+// that outgrows them is built again with the large ones.  The arithmetic does
+// not depend on the limits, so a cell that fits both gets the same bits.
+template <int F, int V, int P, class I>
+struct Limits {
+    static constexpr int FMAX = F;
+    static constexpr int VMAX = V;
+    static constexpr int PMAX = P;
+    using Idx = I;
+};
+using SmallLimits = Limits<48, 32, 96, uint8_t>;
+using LargeLimits = Limits<384, 256, 2048, uint16_t>;
 
 struct V3 {
     double x, y, z;
@@ -40,17 +55,33 @@ struct V3 {
 
 // Faces are rings of indices into a vertex pool; clipping writes the new
 // pool into the other half of v[2][] and flips `cur`.
+template <class LM>
 struct Cell {
+    using Idx = typename LM::Idx;
     int nf;
     int nvert;
     int cur;
     int overflow;
     double rmax2;
-    int fid[FMAX];
-    double fh[FMAX];
-    unsigned char nv[FMAX];
-    unsigned char ring[FMAX][VMAX];
-    V3 v[2][PMAX];
+    int fid[LM::FMAX];
+    double fh[LM::FMAX];
+    uint16_t nv[LM::FMAX];
+    Idx ring[LM::FMAX][LM::VMAX];
+    V3 v[2][LM::PMAX];
+};
+
+// Working arrays of one clip (kept apart from the cell so that large-limit
+// cells can live in a heap or device workspace).
+template <class LM>
+struct ClipScratch {
+    using Idx = typename LM::Idx;
+    double s[LM::PMAX];
+    Idx map[LM::PMAX];
+    uint32_t edge[LM::PMAX];
+    Idx cap[LM::FMAX * 2];
+    Idx out[LM::VMAX];
+    Idx u[LM::VMAX];
+    double pa[LM::VMAX], pb[LM::VMAX];
 };
 
 inline double dot3(const V3& a, const V3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
@@ -59,7 +90,8 @@ inline V3 cross3(const V3& a, const V3& b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 
-void init_box(Cell& c, const double* lo, const double* hi) {
+template <class LM>
+void init_box(Cell<LM>& c, const double* lo, const double* hi) {
     // the bounding box [lo, hi] of all points (plus margin), relative to the
     // cell's point; corner k has coordinates (bit 2, bit 1, bit 0) of k
     const unsigned char q[6][4] = {{4, 6, 7, 5}, {0, 1, 3, 2}, {2, 3, 7, 6},
@@ -100,30 +132,36 @@ inline bool same3(const V3& a, const V3& b) { return a.x == b.x && a.y == b.y &&
 // interpolated from its inner end, shared by the two faces of the edge.  The
 // cap (the new face) is the set of distinct cap vertices, in first-seen order
 // over the faces, sorted by angle around their mean.
-void clip(Cell& c, const V3& n, int nid) {
+template <class LM>
+void clip(Cell<LM>& c, ClipScratch<LM>& w, const V3& n, int nid) {
+    using Idx = typename LM::Idx;
+    constexpr int FMAX = LM::FMAX, VMAX = LM::VMAX, PMAX = LM::PMAX;
+    constexpr Idx NONE = (Idx)~(Idx)0;
+    constexpr uint32_t NOEDGE = 0xffffffffu;
+    constexpr int SH = 8 * (int)sizeof(Idx);
     const double cc = 0.5 * dot3(n, n);
     const double tol = 1e-10 * cc;
     const V3* __restrict__ ov = c.v[c.cur];
     V3* __restrict__ nvx = c.v[c.cur ^ 1];
-    double s[PMAX];
+    double* s = w.s;
     for (int i = 0; i < c.nvert; i++) s[i] = dot3(n, ov[i]) - cc;
     bool any_out = false;
     for (int f = 0; f < c.nf && !any_out; f++)
         for (int k = 0; k < c.nv[f]; k++)
             if (s[c.ring[f][k]] > tol) { any_out = true; break; }
     if (!any_out) return;
-    // new-pool index of every kept old vertex (255 = not yet placed), and for
-    // each new vertex the old edge it was cut from (0xffff for kept vertices)
-    unsigned char map[PMAX];
-    unsigned short edge[PMAX];
-    for (int i = 0; i < c.nvert; i++) map[i] = 255;
+    // new-pool index of every kept old vertex (NONE = not yet placed), and for
+    // each new vertex the old edge it was cut from (NOEDGE for kept vertices)
+    Idx* map = w.map;
+    uint32_t* edge = w.edge;
+    for (int i = 0; i < c.nvert; i++) map[i] = NONE;
     int nn = 0;
-    unsigned char cap[FMAX * 2];
+    Idx* cap = w.cap;
     int ncap = 0;
     int nf = 0;
     for (int f = 0; f < c.nf; f++) {
         const int k0 = c.nv[f];
-        unsigned char out[VMAX];
+        Idx* out = w.out;
         int no = 0;
         for (int k = 0; k < k0; k++) {
             const int ia = c.ring[f][k];
@@ -131,11 +169,11 @@ void clip(Cell& c, const V3& n, int nid) {
             const double sa = s[ia];
             const double sb = s[ib];
             if (sa <= tol) {
-                if (map[ia] == 255) {
+                if (map[ia] == NONE) {
                     if (nn >= PMAX) { c.overflow = 1; return; }
                     nvx[nn] = ov[ia];
-                    edge[nn] = 0xffff;
-                    map[ia] = (unsigned char)nn++;
+                    edge[nn] = NOEDGE;
+                    map[ia] = (Idx)nn++;
                 }
                 if (no < VMAX) out[no++] = map[ia]; else c.overflow = 1;
                 if (sa >= -tol) {
@@ -145,7 +183,7 @@ void clip(Cell& c, const V3& n, int nid) {
             if ((sa < -tol && sb > tol) || (sa > tol && sb < -tol)) {
                 const int iu = sa < 0.0 ? ia : ib;
                 const int iw = sa < 0.0 ? ib : ia;
-                const unsigned short key = (unsigned short)(iu << 8 | iw);
+                const uint32_t key = ((uint32_t)iu << SH) | (uint32_t)iw;
                 int ix = -1;
                 for (int t = 0; t < nn; t++)
                     if (edge[t] == key) { ix = t; break; }
@@ -160,14 +198,14 @@ void clip(Cell& c, const V3& n, int nid) {
                     edge[nn] = key;
                     ix = nn++;
                 }
-                if (no < VMAX) out[no++] = (unsigned char)ix; else c.overflow = 1;
-                if (ncap < FMAX * 2) cap[ncap++] = (unsigned char)ix; else c.overflow = 1;
+                if (no < VMAX) out[no++] = (Idx)ix; else c.overflow = 1;
+                if (ncap < FMAX * 2) cap[ncap++] = (Idx)ix; else c.overflow = 1;
             }
         }
         if (no >= 3) {
             c.fid[nf] = c.fid[f];
             c.fh[nf] = c.fh[f];
-            c.nv[nf] = (unsigned char)no;
+            c.nv[nf] = (uint16_t)no;
             for (int k = 0; k < no; k++) c.ring[nf][k] = out[k];
             nf++;
         }
@@ -175,7 +213,7 @@ void clip(Cell& c, const V3& n, int nid) {
     c.cur ^= 1;
     c.nvert = nn;
     // distinct cap points (by value)
-    unsigned char u[VMAX];
+    Idx* u = w.u;
     int nu = 0;
     for (int i = 0; i < ncap; i++) {
         bool dup = false;
@@ -192,14 +230,15 @@ void clip(Cell& c, const V3& n, int nid) {
         m.x /= nu; m.y /= nu; m.z /= nu;
         const V3 e1 = sub3(nvx[u[0]], m);
         const V3 e2 = cross3(n, e1);
-        double pa[VMAX], pb[VMAX];
+        double* pa = w.pa;
+        double* pb = w.pb;
         for (int i = 0; i < nu; i++) {
             const V3 d = sub3(nvx[u[i]], m);
             pa[i] = dot3(d, e1);
             pb[i] = dot3(d, e2);
         }
         for (int i = 1; i < nu; i++) {   // insertion sort by angle
-            const unsigned char tv = u[i];
+            const Idx tv = u[i];
             const double ta = pa[i], tb = pb[i];
             int j = i - 1;
             while (j >= 0 && ang_less(ta, tb, pa[j], pb[j])) {
@@ -211,7 +250,7 @@ void clip(Cell& c, const V3& n, int nid) {
         if (nf < FMAX) {
             c.fid[nf] = nid;
             c.fh[nf] = std::sqrt(dot3(n, n));
-            c.nv[nf] = (unsigned char)nu;
+            c.nv[nf] = (uint16_t)nu;
             for (int k = 0; k < nu; k++) c.ring[nf][k] = u[k];
             nf++;
         } else {
@@ -228,13 +267,15 @@ void clip(Cell& c, const V3& n, int nid) {
     c.rmax2 = r2;
 }
 
-bool has_box_face(const Cell& c) {
+template <class LM>
+bool has_box_face(const Cell<LM>& c) {
     for (int f = 0; f < c.nf; f++)
         if (c.fid[f] < 0) return true;
     return false;
 }
 
-double cell_volume(const Cell& c) {
+template <class LM>
+double cell_volume(const Cell<LM>& c) {
     const V3* v = c.v[c.cur];
     double vol = 0.0;
     for (int f = 0; f < c.nf; f++) {
@@ -291,7 +332,12 @@ Hash build_hash(const double* p, long np) {
     return H;
 }
 
-double point_volume(const double* p, long i, const Hash& H, Cell& c) {
+// Volume of point i's cell.  With the final (large) limits a cell that still
+// has a box face when it outgrows them is taken as unbounded (0, overflow
+// cleared) and a closed cell's overflow stays set; with the first (small)
+// limits any overflow is left set so that the cell is built again.
+template <class LM>
+double point_volume(const double* p, long i, const Hash& H, Cell<LM>& c, ClipScratch<LM>& w, bool final_pass) {
     const V3 pi = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     int c3[3];
     for (int a = 0; a < 3; a++) {
@@ -320,11 +366,11 @@ double point_volume(const double* p, long i, const Hash& H, Cell& c) {
                         if (d2 == 0.0) continue;                 // duplicate point: same vertex
                         if (0.25 * d2 >= c.rmax2) continue;
                         const bool box = has_box_face(c);
-                        clip(c, n, j);
+                        clip(c, w, n, j);
                         if (c.overflow) {
                             // a cell still reaching the bounding cube that outgrows the limits
                             // is taken as unbounded (0); a closed cell's overflow stays an error
-                            if (box) c.overflow = 0;
+                            if (box && final_pass) c.overflow = 0;
                             return 0.0;
                         }
                     }
@@ -346,13 +392,26 @@ long orc_voronoi_volumes(const double* pts, long np, double* vol, int nthreads) 
     long over = 0;
 #pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads()) reduction(+ : over)
     {
-        vor::Cell* c = new vor::Cell;
+        auto* cs = new vor::Cell<vor::SmallLimits>;
+        auto* ws = new vor::ClipScratch<vor::SmallLimits>;
+        vor::Cell<vor::LargeLimits>* cl = nullptr;
+        vor::ClipScratch<vor::LargeLimits>* wl = nullptr;
 #pragma omp for schedule(dynamic, 256)
         for (long i = 0; i < np; i++) {
-            vol[i] = vor::point_volume(pts, i, H, *c);
-            over += c->overflow;
+            vol[i] = vor::point_volume(pts, i, H, *cs, *ws, false);
+            if (cs->overflow) {   // second pass with the large limits
+                if (!cl) {
+                    cl = new vor::Cell<vor::LargeLimits>;
+                    wl = new vor::ClipScratch<vor::LargeLimits>;
+                }
+                vol[i] = vor::point_volume(pts, i, H, *cl, *wl, true);
+                over += cl->overflow;
+            }
         }
-        delete c;
+        delete cs;
+        delete ws;
+        delete cl;
+        delete wl;
     }
     return over;
 }

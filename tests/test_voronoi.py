@@ -60,6 +60,43 @@ def test_wave_caipi_interior_matches_qhull(orc):
     np.testing.assert_allclose(a[both], b[both], rtol=1e-10)
 
 
+def sparse_bin_points(n_lines=None):
+    """Samples of the sparsest respiratory bin of the 128x128x96 4D workload
+    (7 % of the R=2x2 lines): long, thin cells with many faces, which outgrow
+    the small cell limits and are rebuilt with the large ones."""
+    cfg = synth.CONFIGS["4d_128x128x96"]
+    grid, samp = cfg["grid"], cfg["samp"]
+    traj = synth.wave_caipi_trajectory(grid, samp, cfg["cycles"], cfg["amp"])
+    lines = synth.caipi_mask(samp[1], samp[2], cfg["R"], cfg["shift"])
+    labels = synth.respiratory_bins(lines, cfg["bins"], cfg["bin_frac"], cfg["seed"] + 5)
+    b = int(np.argmin(np.bincount(labels[labels >= 0])))
+    return traj[synth.sample_mask_from_lines(labels == b, samp[0])]
+
+
+def test_dense_cluster_interior_matches_qhull(orc):
+    # a dense cluster inside a uniform background: the cluster's interior
+    # cells outgrow the small limits while still reaching the bounding box
+    # early in their construction; built again with the large limits they
+    # must come out closed and equal to Qhull's volumes
+    from oracle import voronoi_ref
+
+    rng = np.random.default_rng(0)
+    p = np.concatenate([rng.normal(0.5, 0.05, (5000, 3)), rng.uniform(0, 1, (3000, 3))])
+    a = orc.voronoi_volumes(p)
+    b = voronoi_ref.voronoi_volumes(p)
+    both = (a > 0) & (b > 0)
+    np.testing.assert_allclose(a[both], b[both], rtol=1e-10)
+    assert np.all(a[b == 0] == 0)
+    core = np.linalg.norm(p - 0.5, axis=1) < 0.1
+    assert np.all(a[core] > 0)
+
+
+def test_sparse_bin_cells_fit_the_large_limits(orc):
+    p = sparse_bin_points()
+    vol = orc.voronoi_volumes(p)          # raises if a closed cell outgrew the limits
+    assert (vol > 0).mean() > 0.75
+
+
 def test_density_invariant_to_translation_and_scale(orc):
     rng = np.random.default_rng(1)
     p = rng.uniform(0, 1, (1500, 3))
@@ -69,12 +106,14 @@ def test_density_invariant_to_translation_and_scale(orc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["uniform", "lattice", "wave"])
+@pytest.mark.parametrize("case", ["uniform", "lattice", "wave", "sparse_bin"])
 def test_device_voronoi_bit_exact(pkg, orc, case):
     if case == "uniform":
         p = np.random.default_rng(2).uniform(0, 1, (20000, 3))
     elif case == "lattice":
         p = lattice(10)
+    elif case == "sparse_bin":
+        p = sparse_bin_points()
     else:
         grid, samp = (128, 128, 96), (128, 64, 48)
         traj = synth.wave_caipi_trajectory(grid, samp, 3, 4.0)
