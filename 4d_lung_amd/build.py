@@ -75,7 +75,7 @@ def build_reco():
     C-ABI) and the per-state driver program."""
     flags = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
              "-I" + INCLUDE, "-I" + RECO_INC]
-    srcs = [os.path.join(RECO_SRC, f) for f in ("Reco.cpp", "Fourier.cpp", "IO.cpp", "Trajectory.cpp")]
+    srcs = [os.path.join(RECO_SRC, f) for f in ("Reco.cpp", "Fourier.cpp", "IO.cpp", "Trajectory.cpp", "functions.cpp")]
     tmp = RECO_LIB + ".tmp"
     _run([CXX] + flags + ["-shared", "-o", tmp] + srcs +
          ["-L" + HERE, "-l:libcgs_hip.so", "-Wl,-rpath,$ORIGIN"])

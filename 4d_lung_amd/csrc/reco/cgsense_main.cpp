@@ -4,7 +4,8 @@
 //
 //   cgsense_dropin DIR
 //   DIR/params.txt : gridX gridY gridZ sX sY sZ nCh cgThreshold maxIter density nStates traj
-//                    density: 0 none, 2 import (DIR/dcf.bin); traj: 0 DIR/traj.bin
+//                    density: 0 none, 1 Voronoi (estimateDensityV3D over the whole
+//                    trajectory), 2 import (DIR/dcf.bin); traj: 0 DIR/traj.bin
 //                    (nS x 3 doubles, memory order), 1 wave-CAIPI (DIR/wave.txt:
 //                    nOsc dwell gradAmp ampCorrY ampCorrZ shiftY shiftZ fovY fovZ)
 //   DIR/state{n}.bin, DIR/sensitivityMaps/map{c}.bin
@@ -23,6 +24,7 @@
 #include "IO.h"
 #include "Reco.h"
 #include "Trajectory.h"
+#include "functions.h"
 
 class FileTrajectory : public Trajectory {
 public:
@@ -68,7 +70,9 @@ int main(int argc, char** argv) {
     reco.setOutputParentDir(dir);
     reco.setCGThreshold(thr);
     reco.setMaxIterations(maxIter);
-    reco.setDensityCorrection(dens == 2 ? Reco::Density::IMPORT : Reco::Density::NONE);
+    reco.setDensityCorrection(dens == 2   ? Reco::Density::IMPORT
+                              : dens == 1 ? Reco::Density::VORONOI3D
+                                          : Reco::Density::NONE);
     reco.setDensityCorrectionPath(dir + "dcf.bin");
 
     FileTrajectory* ftj = nullptr;
@@ -131,6 +135,8 @@ int main(int argc, char** argv) {
     reco.kernel->calculateWeights();
     if (reco.getDensityCorrection() == Reco::Density::IMPORT)
         reco.denCorr = IO::importDensityCorr(reco.getDensityCorrectionPath(), nSamplesPerCh);
+    else if (reco.getDensityCorrection() == Reco::Density::VORONOI3D)
+        reco.denCorr = estimateDensityV3D(reco.trajectory->kSpaceTraj, sx, sy, sz, false, 1.0);
     for (int n = 0; n < nStates; n++) {
         if (n > 0) {
             IO::importDataBin(reco, reco.data, reco.getBatchFileNum(n));
@@ -138,21 +144,10 @@ int main(int argc, char** argv) {
             reco.kernel->calculateWeights();
         }
         reco.kernel->excludeMissingData();
-        // performDensityCorrection (functions.h:319-335)
-        if (!reco.denCorr.empty())
-            for (int c = 0; c < nch; c++)
-                for (int i = 0; i < nSamplesPerCh; i++) {
-                    reco.data[(size_t)c * nSamplesPerCh + i][0] *= reco.denCorr[i];
-                    reco.data[(size_t)c * nSamplesPerCh + i][1] *= reco.denCorr[i];
-                }
+        performDensityCorrection(reco, reco.data, reco.denCorr, nSamplesPerCh, nch);
         reco.allocImg();
         reco.nufftBWD();
-        // root-sum-of-squares of the coil images (functions.h:290-316)
-        std::vector<float> rss(cGridSize, 0.0f);
-        for (int c = 0; c < nch; c++)
-            for (int i = 0; i < cGridSize; i++) rss[i] += reco.img[c][i][0] * reco.img[c][i][0] + reco.img[c][i][1] * reco.img[c][i][1];
-        for (auto& v : rss) v = std::sqrt(v);
-        IO::exportVectorBin(rss, dir + "resultRssq.bin");
+        IO::exportVectorBin(rssq2DVecImg(reco.img, cGridSize), dir + "resultRssq.bin");
         if (n == 0) IO::importSensitivityMaps(reco.sMaps, cGridSize, nch, dir + "sensitivityMaps/");
         reco.cgSenseDynamic();
         reco.finalize();

@@ -18,19 +18,21 @@ GAMMA_BAR = 4.257748059639612e7
 
 
 def write_case(d, P, thr=0.0, max_iter=9, dens=False, nstates=1, traj_kind=0):
+    """dens: False (none), True (import DIR/dcf.bin) or "voronoi"."""
     grid, samp, C = P["grid"], P["samp"], P["ncoil"]
     os.makedirs(os.path.join(d, "sensitivityMaps"), exist_ok=True)
     os.makedirs(os.path.join(d, "results"), exist_ok=True)
+    mode = 1 if dens == "voronoi" else (2 if dens else 0)
     with open(os.path.join(d, "params.txt"), "w") as f:
         f.write("%d %d %d %d %d %d %d %.17g %d %d %d %d\n" % (tuple(grid) + tuple(samp) + (
-            C, thr, max_iter, 2 if dens else 0, nstates, traj_kind)))
+            C, thr, max_iter, mode, nstates, traj_kind)))
     P["traj"].astype(np.float64).tofile(os.path.join(d, "traj.bin"))
     # state file: coil slowest, then z line, y line, readout fastest (IO.cpp:85-93)
     file_order = P["data"].reshape((C,) + tuple(samp)).transpose(0, 3, 2, 1)
     np.ascontiguousarray(file_order).astype(np.complex64).tofile(os.path.join(d, "state0.bin"))
     for c in range(C):
         P["maps"][c].astype(np.complex64).tofile(os.path.join(d, "sensitivityMaps", "map%d.bin" % c))
-    if dens:
+    if dens is True:
         P["dcf"].astype(np.float32).tofile(os.path.join(d, "dcf.bin"))
 
 
@@ -145,9 +147,14 @@ def test_wave_caipi_trajectory(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dens", [False, True])
+@pytest.mark.parametrize("dens", [False, True, "voronoi"])
 def test_dropin_main_matches_oracle(orc, tmp_path, dens):
-    P = small_problem(orc, seed=22, dcf=dens)
+    P = small_problem(orc, seed=22, dcf=dens is True)
+    if dens == "voronoi":
+        # Density::VORONOI3D: estimateDensityV3D over the whole trajectory
+        # (functions.h:161-258), bit-exact with the oracle's statement
+        P["dcf"] = orc.voronoi_density(P["traj"])
+        P["oracle"].set_density(P["dcf"])
     d = str(tmp_path)
     write_case(d, P, dens=dens, max_iter=9)
     out = run_driver(d)
