@@ -262,6 +262,8 @@ int setup_kb(cgs_ctx* h) {
     const double v1 = host_kb_axis(0.0f, kb.ax[1], kb.b);
     const double v2 = host_kb_axis(0.0f, kb.ax[2], kb.b);
     kb.thr = (double)(float)((v0 * v1) * v2) * 1e-7;   // Reco.cpp:716-717
+    kb.thr_up = (float)kb.thr;
+    if ((double)kb.thr_up <= kb.thr) kb.thr_up = std::nextafter(kb.thr_up, INFINITY);
     int hw = (int)std::floor((double)wHalf + 0.5 + 1e-4);
     if (hw > h->order) hw = h->order;
     kb.h = hw;
@@ -586,6 +588,7 @@ GridArgs grid_args(cgs_ctx* h, int b0, int nb) {
     g.tile_off = h->p_tile_off;
     g.gdcf = nullptr;
     g.thr = h->kb.thr;
+    g.thrf = h->kb.thr_up;
     g.b0 = b0;
     g.nb = nb;
     g.done = nullptr;
@@ -1424,6 +1427,7 @@ GridArgs pts_grid_args(cgs_pts* p) {
     g.tile_off = p->tile_off;
     g.gdcf = nullptr;
     g.thr = h->kb.thr;
+    g.thrf = h->kb.thr_up;
     g.b0 = 0;
     g.nb = 1;
     g.done = nullptr;
@@ -1443,6 +1447,7 @@ int pts_grad(cgs_pts* p, const float2* b_caller, float scale, float* gk) {
     a.m = p->m;
     a.g = geom(h);
     a.thr = h->kb.thr;
+    a.thrf = h->kb.thr_up;
     a.scale = scale;
     a.gk = gk;
     {

@@ -301,10 +301,24 @@ __device__ __forceinline__ void fft_last_stage(const float2* lds, const float2* 
             const int col = b % NCOL, j = b / NCOL;
             Dft<R, DIR>::run(v[p]);
 #pragma unroll
-            for (int i = 0; i < R; ++i) st(j + i * LS, col, v[p][i]);
+            for (int i = 0; i < R; ++i) st(j + i * LS, col, v[p][i], p, i);
         }
     }
 }
+
+// Output slot (p, i) of the last stage: line and column, as fft_last_stage
+// hands them to its store functor (for epilogue operands fetched early).
+template <int L, int NCOL, int NT>
+struct LastStageMap {
+    static constexpr int NS = plan_nstages(L);
+    static constexpr int R = plan_radix(L, NS - 1);
+    static constexpr int LS = plan_ls(L, NS - 1);
+    static constexpr int NB = NCOL * (L / R);
+    static constexpr int PER = (NB + NT - 1) / NT;
+    __device__ static __forceinline__ bool valid(int p) { return (NB % NT) == 0 || (int)threadIdx.x + p * NT < NB; }
+    __device__ static __forceinline__ int col(int p) { return ((int)threadIdx.x + p * NT) % NCOL; }
+    __device__ static __forceinline__ int line(int p, int i) { return ((int)threadIdx.x + p * NT) / NCOL + i * LS; }
+};
 
 // Line transform of an [L x NCOL] tile already in LDS (and visible to the
 // block): every stage but the last through LDS, the last into st.
@@ -337,7 +351,7 @@ __device__ __forceinline__ void fft_line_tile(float2* lds, const float2* tw, Loa
                 for (int i = 0; i < R; ++i) v[i] = ld(i, col);
                 Dft<R, DIR>::run(v);
 #pragma unroll
-                for (int i = 0; i < R; ++i) st(i, col, v[i]);
+                for (int i = 0; i < R; ++i) st(i, col, v[i], p, i);
             }
         }
     } else {
@@ -498,7 +512,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
                     if (W <= nh || c < nh) v = *elem<Off>(g, (Off)l * ls + (Off)c);
                     return alt_sign(v, l);
                 },
-                [&](int l, int c, float2 v) {
+                [&](int l, int c, float2 v, int, int) {
                     if (W <= nh || c < nh) *elem<Off>(g, (Off)l * ls + (Off)c) = alt_sign(v, l);
                 });
         }
@@ -525,7 +539,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
         if (chunk + 1 < c_end) load_chunk(next, chunk + 1);
         fft_mid_stages<L, 1, NS - 1, plan_radix(L, 0), DIR, W, W, CGS_THREADS>(lds, twl);
         float2* g = gout + chunk * W;
-        fft_last_stage<L, DIR, W, W, false, CGS_THREADS>(lds, twl, [&](int l, int c, float2 v) {
+        fft_last_stage<L, DIR, W, W, false, CGS_THREADS>(lds, twl, [&](int l, int c, float2 v, int, int) {
             if (c <= cmax) *elem<Off>(g, (Off)l * ls + (Off)c) = alt_sign(v, l);
         });
         if (chunk + 1 < c_end) __syncthreads();
@@ -602,7 +616,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
         next.template to_lds<W>(lds, cv);
         __syncthreads();
         if (c0 + W < C) next.load(ld(c0 + W));
-        fft_lds_tile<L, -1, W, W, false, NT>(lds, twl, [&](int l, int c, float2 v) {
+        fft_lds_tile<L, -1, W, W, false, NT>(lds, twl, [&](int l, int c, float2 v, int, int) {
             const int coil = c0 + c;
             if (coil < C) g[l * C + coil] = alt_sign(v, l);
         });
@@ -658,18 +672,33 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
         next.template to_lds<LDW>(lds, cv);
         __syncthreads();
         if (c0 + W < C) next.load(ld(c0 + W));
+        // the epilogue's operands (S_c and kernelFT of every output of this
+        // thread's last-stage butterflies) are fetched now, while the stages run
+        using LM = LastStageMap<L, W, NT>;
+        float2 spre[LM::PER][LM::R];
+        float kpre[LM::PER][LM::R];
+#pragma unroll
+        for (int p = 0; p < LM::PER; ++p) {
+            if (!LM::valid(p)) continue;
+            const int coil = min(c0 + LM::col(p), C - 1);
+#pragma unroll
+            for (int i = 0; i < LM::R; ++i) {
+                const int l = LM::line(p, i);
+                spre[p][i] = ml ? ml[l * C + coil] : make_float2(0.0f, 0.0f);
+                kpre[p][i] = kl ? kl[l] : 1.0f;
+            }
+        }
         // inverse transform; the epilogue runs on the last stage's outputs:
         // products with conj(S) go back into the tile for the coil sum, coil
         // images go straight to the grid
-        fft_lds_tile<L, +1, W, LDW, true, NT>(lds, twl, [&](int l, int c, float2 v) {
+        fft_lds_tile<L, +1, W, LDW, true, NT>(lds, twl, [&](int l, int c, float2 v, int p, int i) {
             const int coil = c0 + c;
             float2 r = make_float2(0.0f, 0.0f);
             if (coil < C) {
                 r = alt_sign(v, l);
-                const float sc = kl ? kl[l] * a.scale : a.scale;
-                r = cscale(r, sc);
+                r = cscale(r, kpre[p][i] * a.scale);
                 if (!xy_in || (a.crop && (l < a.lo[2] || l > a.hi[2]))) r = make_float2(0.0f, 0.0f);
-                if (combine) r = cmulc(ml[l * C + coil], r);
+                if (combine) r = cmulc(spre[p][i], r);
                 else g[l * C + coil] = r;
             }
             if (combine) lds[l * LDW + c] = r;

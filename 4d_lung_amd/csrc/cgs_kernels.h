@@ -58,6 +58,7 @@ struct KbParams {
     AxisConst ax[3];
     double b;      // double(float b)
     double thr;    // double(float(convFunction(0))) * 1e-7 (Reco.cpp:716-717)
+    float thr_up;  // smallest float above thr: for a float w, w >= thr_up <=> double(w) > thr
     int h;         // window half width (taps at nearest-h .. nearest+h)
     int kw;        // 2h+1
     int order;     // reference search order (Reco.cpp:380)
@@ -104,6 +105,7 @@ struct GridArgs {
     const int* tile_off;   // [nbins_total][ntiles+1] absolute sample offsets
     const float* gdcf;     // gridding only: multiply Y by these weights first (or null)
     double thr;
+    float thrf;            // KbParams::thr_up
     int b0, nb;
     const int* done;       // [nbins_total] flags, or null
 };
@@ -125,6 +127,7 @@ struct GradKArgs {
     long long m;
     Geom g;
     double thr;
+    float thrf;            // KbParams::thr_up
     float scale;
     float* gk;             // [m][3], original order
 };

@@ -59,7 +59,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
     const int C = a.g.C;
     const float2* __restrict__ G = a.G + (long long)(bin - a.b0) * a.g.ng * C;
     const float d = a.s.dcf ? a.s.dcf[s] : 1.0f;
-    const double thr = a.thr;
+    const float thrf = a.thrf;
     for (int c = lane_c; c < C; c += CW) {
         float2 acc = make_float2(0.0f, 0.0f);
         for (int ox = 0; ox < KW; ++ox) {
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
 #pragma unroll
                 for (int oz = 0; oz < KW; ++oz) {
                     float w = (float)(vxy * vz[oz]);
-                    w = (double)w > thr ? w : 0.0f;
+                    w = w >= thrf ? w : 0.0f;
                     acc.x = fmaf(w, gv[oy][oz].x, acc.x);
                     acc.y = fmaf(w, gv[oy][oz].y, acc.y);
                 }
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
                                     const int oz = z0 + z - bs.z;
                                     if (oz >= 0 && oz < KW) {
                                         const float w = (float)(vxy * v[2 * KW + oz]);
-                                        if ((double)w > a.thr) {
+                                        if (w >= a.thrf) {
                                             acc[k][z].x = fmaf(w, yv.x, acc[k][z].x);
                                             acc[k][z].y = fmaf(w, yv.y, acc[k][z].y);
                                         }
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grad_k(GradKArgs a) {
                         const int zg = bs.z + oz;
                         if (zg < 0 || zg >= a.g.nz) continue;
                         const float w = (float)(vxy * v[2 * KW + oz]);
-                        if (!((double)w > a.thr)) continue;
+                        if (!(w >= a.thrf)) continue;
                         const float2 av = a.A[(row + zg) * C + c];
                         const float t = bv.x * av.x + bv.y * av.y;
                         g0 = fmaf(t, (float)(dxy * v[2 * KW + oz]), g0);
