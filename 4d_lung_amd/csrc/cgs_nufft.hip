@@ -90,11 +90,14 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
 }
 
 // -------------------------------------------------------------- gridding
-template <int KW, int TX, int TY, int TZ, int CP>
+// A thread owns CG coils of one column (x, y) of the tile and its TZ cells
+// along z: each tap weight is formed once per CG coils.
+template <int KW, int TX, int TY, int TZ, int CP, int CG>
 __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     constexpr int VS = val_stride(KW);
     constexpr int NCOL = TX * TY;
-    constexpr int NPAIR = NCOL * CP;
+    constexpr int NG = CP / CG;              // coil groups
+    constexpr int NPAIR = NCOL * NG;
     constexpr int NPT = (NPAIR + CGS_THREADS - 1) / CGS_THREADS;
     constexpr int NWAVE = CGS_THREADS / 64;
     constexpr int SUB = 64;                  // candidates staged in LDS at a time
@@ -117,11 +120,13 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     const int C = a.g.C;
     const int* __restrict__ off = a.tile_off + (long long)bin * (ntiles + 1);
 
-    float2 acc[NPT][TZ];
+    float2 acc[NPT][CG][TZ];
 #pragma unroll
     for (int k = 0; k < NPT; ++k)
 #pragma unroll
-        for (int z = 0; z < TZ; ++z) acc[k][z] = make_float2(0.0f, 0.0f);
+        for (int q = 0; q < CG; ++q)
+#pragma unroll
+            for (int z = 0; z < TZ; ++z) acc[k][q][z] = make_float2(0.0f, 0.0f);
 
     const int tid = threadIdx.x;
     const int lane = tid % 64, wid = tid / 64;
@@ -200,22 +205,28 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
 #pragma unroll
                 for (int k = 0; k < NPT; ++k) {
                     const int e = tid + k * CGS_THREADS;
-                    const int c = e % CP, col = e / CP;
-                    if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
+                    const int g = e % NG, col = e / NG;
+                    if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && g * CG < C) {
                         const int ox = x0 + col / TY - bs.x;
                         const int oy = y0 + col % TY - bs.y;
                         if (ox >= 0 && ox < KW && oy >= 0 && oy < KW) {
                             const double vxy = v[ox] * v[KW + oy];
                             if (vxy != 0.0) {
-                                const float2 yv = sy[j * CP + c];
+                                // staged values of coils >= C are 0
+                                float2 yv[CG];
+#pragma unroll
+                                for (int q = 0; q < CG; ++q) yv[q] = sy[j * CP + g * CG + q];
 #pragma unroll
                                 for (int z = 0; z < TZ; ++z) {
                                     const int oz = z0 + z - bs.z;
                                     if (oz >= 0 && oz < KW) {
                                         const float w = (float)(vxy * v[2 * KW + oz]);
                                         if (w >= a.thrf) {
-                                            acc[k][z].x = fmaf(w, yv.x, acc[k][z].x);
-                                            acc[k][z].y = fmaf(w, yv.y, acc[k][z].y);
+#pragma unroll
+                                            for (int q = 0; q < CG; ++q) {
+                                                acc[k][q][z].x = fmaf(w, yv[q].x, acc[k][q][z].x);
+                                                acc[k][q][z].y = fmaf(w, yv[q].y, acc[k][q][z].y);
+                                            }
                                         }
                                     }
                                 }
@@ -232,14 +243,18 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
         const int e = tid + k * CGS_THREADS;
-        const int c = e % CP, col = e / CP;
-        if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && c < C) {
+        const int g = e % NG, col = e / NG;
+        if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && g * CG < C) {
             const int x = x0 + col / TY, y = y0 + col % TY;
             if (x < a.g.nx && y < a.g.ny) {
                 const long long row = ((long long)x * a.g.ny + y) * a.g.nz;
 #pragma unroll
                 for (int z = 0; z < TZ; ++z)
-                    if (z0 + z < a.g.nz) G[(row + z0 + z) * C + c] = acc[k][z];
+                    if (z0 + z < a.g.nz) {
+#pragma unroll
+                        for (int q = 0; q < CG; ++q)
+                            if (g * CG + q < C) G[(row + z0 + z) * C + g * CG + q] = acc[k][q][z];
+                    }
             }
         }
     }
@@ -334,8 +349,10 @@ hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Gridding tile shapes (cells per tile along x, y, z).
-static const int kTileCfg[][3] = {{8, 8, 8}, {8, 8, 4}, {4, 8, 8}, {4, 4, 8}};
+// Gridding tile shapes: cells per tile along x, y, z and coils per thread
+// (at most; fewer when the padded coil count is smaller).
+static const int kTileCfg[][4] = {{8, 8, 8, 1}, {8, 8, 4, 1}, {4, 8, 8, 1}, {4, 4, 8, 1},
+                                  {8, 8, 8, 4}, {8, 8, 4, 4}, {4, 8, 8, 2}, {8, 8, 8, 2}};
 
 int grid_tile_configs() { return (int)(sizeof(kTileCfg) / sizeof(kTileCfg[0])); }
 
@@ -345,14 +362,16 @@ void grid_tile_dims(int tcfg, int* tx, int* ty, int* tz) {
     *tz = kTileCfg[tcfg][2];
 }
 
-template <int KW, int TX, int TY, int TZ>
+template <int KW, int TX, int TY, int TZ, int CG>
 static hipError_t grid_cp(const GridArgs& a, hipStream_t s) {
     const int cp = pow2_at_least(a.g.C, 64);
     const dim3 grid((unsigned)((long long)a.nb * a.t.ntiles));
     switch (cp) {
-        case 1: case 2: case 4: case 8: k_grid<KW, TX, TY, TZ, 8><<<grid, CGS_THREADS, 0, s>>>(a); break;
-        case 16: k_grid<KW, TX, TY, TZ, 16><<<grid, CGS_THREADS, 0, s>>>(a); break;
-        case 32: k_grid<KW, TX, TY, TZ, 32><<<grid, CGS_THREADS, 0, s>>>(a); break;
+        case 1: case 2: case 4: case 8:
+            k_grid<KW, TX, TY, TZ, 8, (CG < 8 ? CG : 8)><<<grid, CGS_THREADS, 0, s>>>(a);
+            break;
+        case 16: k_grid<KW, TX, TY, TZ, 16, CG><<<grid, CGS_THREADS, 0, s>>>(a); break;
+        case 32: k_grid<KW, TX, TY, TZ, 32, CG><<<grid, CGS_THREADS, 0, s>>>(a); break;
         default: return hipErrorInvalidValue;   // more than 32 coils: not supported by gridding
     }
     return hipGetLastError();
@@ -361,10 +380,14 @@ static hipError_t grid_cp(const GridArgs& a, hipStream_t s) {
 template <int KW>
 static hipError_t grid_tile(int tcfg, const GridArgs& a, hipStream_t s) {
     switch (tcfg) {
-        case 0: return grid_cp<KW, 8, 8, 8>(a, s);
-        case 1: return grid_cp<KW, 8, 8, 4>(a, s);
-        case 2: return grid_cp<KW, 4, 8, 8>(a, s);
-        case 3: return grid_cp<KW, 4, 4, 8>(a, s);
+        case 0: return grid_cp<KW, 8, 8, 8, 1>(a, s);
+        case 1: return grid_cp<KW, 8, 8, 4, 1>(a, s);
+        case 2: return grid_cp<KW, 4, 8, 8, 1>(a, s);
+        case 3: return grid_cp<KW, 4, 4, 8, 1>(a, s);
+        case 4: return grid_cp<KW, 8, 8, 8, 4>(a, s);
+        case 5: return grid_cp<KW, 8, 8, 4, 4>(a, s);
+        case 6: return grid_cp<KW, 4, 8, 8, 2>(a, s);
+        case 7: return grid_cp<KW, 8, 8, 8, 2>(a, s);
         default: return hipErrorInvalidValue;
     }
 }

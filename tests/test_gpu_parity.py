@@ -95,6 +95,16 @@ def test_nufft_bwd_matches_oracle(pkg, orc, small):
     assert rel_l2(got, ref) < OP_TOL
 
 
+def test_nufft_bwd_every_tile_config(pkg, orc, small):
+    # every gridding tile shape / coils-per-thread variant the autotuner can pick
+    ref = small["oracle"].nufft_bwd(small["data"].astype(np.complex128), prec=64)
+    for t in range(pkg.cgs.lib().cgs_num_tile_configs()):
+        ctx = gpu_context(pkg, small, tcfg=t)
+        got = ctx.nufft_bwd(small["data"])
+        assert rel_l2(got, ref) < OP_TOL, t
+        ctx.close()
+
+
 def test_nufft_fwd_matches_oracle(pkg, orc, small):
     ctx = gpu_context(pkg, small)
     rng = np.random.default_rng(3)
