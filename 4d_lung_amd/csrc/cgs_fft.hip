@@ -575,42 +575,42 @@ template <int L, int W, int NT, bool HAS_P, bool HAS_K>
 __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     __shared__ float2 lds[L * W];
     __shared__ float2 twl[L];
+    __shared__ float2 pline[HAS_P ? L : 1];   // p along the line (shared by all coils)
+    __shared__ float kline[HAS_K ? L : 1];    // kernelFT along the line
     const long long outer = blockIdx.x;             // bin * nxy + xy
     const long long bin = outer / a.nxy, xy = outer % a.nxy;
     if (a.done && a.done[bin]) return;
-    for (int e = threadIdx.x; e < L; e += NT) twl[e] = a.tw[e];
     const long long cell0 = xy * L;                  // cell index of (x, y, z=0)
+    for (int e = threadIdx.x; e < L; e += NT) {
+        twl[e] = a.tw[e];
+        if constexpr (HAS_P) pline[e] = a.p[bin * a.ng + cell0 + e];
+        if constexpr (HAS_K) kline[e] = a.kft[cell0 + e];
+    }
     // block-uniform bases and 32-bit offsets within the line; coil indices
     // are clamped so that every load is unconditional (columns past the last
     // coil are zeroed when the tile is written to LDS)
     float2* g = a.g + (bin * a.ng + cell0) * a.C;
     const int C = a.C;
-    const float2* pl = HAS_P ? a.p + bin * a.ng + cell0 : nullptr;
     const float2* ml = HAS_P ? a.maps + cell0 * C : nullptr;
-    const float* kl = HAS_K ? a.kft + cell0 : nullptr;
+    // raw element: S (with p) or the coil image value (without)
     auto ld = [&](int c0) {
         return [&, c0](int l, int c) {
             const int coil = min(c0 + c, C - 1);
-            ZRaw r;
-            if constexpr (HAS_P) {
-                r.u = pl[l];
-                r.w = ml[l * C + coil];
-            } else {
-                r.u = g[l * C + coil];
-            }
-            if constexpr (HAS_K) r.k = kl[l];
-            return r;
+            if constexpr (HAS_P) return ml[l * C + coil];
+            else return g[l * C + coil];
         };
     };
     int cur = 0;
-    auto cv = [&](int l, int c, const ZRaw& r) {
-        float2 v = HAS_P ? cmul(r.u, r.w) : r.u;
-        if constexpr (HAS_K) v = cscale(v, r.k);
+    auto cv = [&](int l, int c, float2 r) {
+        float2 v = r;
+        if constexpr (HAS_P) v = cmul(pline[l], r);
+        if constexpr (HAS_K) v = cscale(v, kline[l]);
         if (cur + c >= C) v = make_float2(0.0f, 0.0f);
         return alt_sign(v, l);
     };
-    TileRegs<L, W, NT, ZRaw> next;
+    TileRegs<L, W, NT, float2> next;
     next.load(ld(0));
+    __syncthreads();   // pline / kline / twl visible
     for (int c0 = 0; c0 < C; c0 += W) {
         cur = c0;
         next.template to_lds<W>(lds, cv);
@@ -641,19 +641,22 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
     constexpr int RPT = (L + NT - 1) / NT;
     __shared__ float2 lds[L * LDW];
     __shared__ float2 twl[L];
+    __shared__ float kline[L];      // kernelFT * scale along the line
     __shared__ double red[2 * (NT / 64)];
     const long long outer = blockIdx.x;
     const long long bin = outer / a.nxy, xy = outer % a.nxy;
     if (a.done && a.done[bin]) return;
-    for (int e = threadIdx.x; e < L; e += NT) twl[e] = a.tw[e];
     const long long cell0 = xy * L;
+    for (int e = threadIdx.x; e < L; e += NT) {
+        twl[e] = a.tw[e];
+        kline[e] = a.kft ? a.kft[cell0 + e] * a.scale : a.scale;
+    }
     const int gx = (int)(xy / a.ny), gy = (int)(xy % a.ny);
     const bool xy_in = !a.crop || (gx >= a.lo[0] && gx <= a.hi[0] && gy >= a.lo[1] && gy <= a.hi[1]);
     float2* g = a.g + (bin * a.ng + cell0) * a.C;
     const bool combine = a.maps != nullptr;
     const int C = a.C;
     const float2* ml = a.maps ? a.maps + cell0 * C : nullptr;
-    const float* kl = a.kft ? a.kft + cell0 : nullptr;
     float2 acc[RPT];
 #pragma unroll
     for (int r = 0; r < RPT; ++r) acc[r] = make_float2(0.0f, 0.0f);
@@ -672,21 +675,17 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
         next.template to_lds<LDW>(lds, cv);
         __syncthreads();
         if (c0 + W < C) next.load(ld(c0 + W));
-        // the epilogue's operands (S_c and kernelFT of every output of this
-        // thread's last-stage butterflies) are fetched now, while the stages run
+        // the epilogue's S_c of every output of this thread's last-stage
+        // butterflies is fetched now, while the stages run
         using LM = LastStageMap<L, W, NT>;
         float2 spre[LM::PER][LM::R];
-        float kpre[LM::PER][LM::R];
 #pragma unroll
         for (int p = 0; p < LM::PER; ++p) {
             if (!LM::valid(p)) continue;
             const int coil = min(c0 + LM::col(p), C - 1);
 #pragma unroll
-            for (int i = 0; i < LM::R; ++i) {
-                const int l = LM::line(p, i);
-                spre[p][i] = ml ? ml[l * C + coil] : make_float2(0.0f, 0.0f);
-                kpre[p][i] = kl ? kl[l] : 1.0f;
-            }
+            for (int i = 0; i < LM::R; ++i)
+                spre[p][i] = ml ? ml[LM::line(p, i) * C + coil] : make_float2(0.0f, 0.0f);
         }
         // inverse transform; the epilogue runs on the last stage's outputs:
         // products with conj(S) go back into the tile for the coil sum, coil
@@ -696,7 +695,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
             float2 r = make_float2(0.0f, 0.0f);
             if (coil < C) {
                 r = alt_sign(v, l);
-                r = cscale(r, kpre[p][i] * a.scale);
+                r = cscale(r, kline[l]);
                 if (!xy_in || (a.crop && (l < a.lo[2] || l > a.hi[2]))) r = make_float2(0.0f, 0.0f);
                 if (combine) r = cmulc(spre[p][i], r);
                 else g[l * C + coil] = r;
