@@ -329,6 +329,11 @@ def main():
         b = algorithmic_bytes(name, ng, C, m, vs)
         kernels[name] = dict(count=cnt, avg_ms=avg, gbs=(b / (avg * 1e-3) / 1e9) if b else None, bytes=b)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count"])
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+    # this workload (profiles/traffic_<workload>.txt, scaled by a copy kernel)
+    prof_mod = importlib.import_module("4d_lung_amd.profiling")
+    traffic_src = os.path.join("profiles", "traffic_%s.txt" % args.workload)
+    traffic = prof_mod.read_traffic(os.path.join(ROOT, traffic_src))
     dk = kernels[dom]
     for name in sorted(kernels, key=lambda k: -kernels[k]["avg_ms"]):
         kk = kernels[name]
@@ -352,7 +357,9 @@ def main():
                    "gridding_tile": list(ctx.tile_config()[1])},
         "plain_path_value": plain_value,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (dk["gbs"] / HBM_PEAK_GBS) if dk["gbs"] else None, "traffic": None,
+                     "frac": (dk["gbs"] / HBM_PEAK_GBS) if dk["gbs"] else None,
+                     "traffic": traffic.get(dom, (None, None))[0] + traffic[dom][1] if dom in traffic else None,
+                     "traffic_source": traffic_src if dom in traffic else None,
                      "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"]},
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "gbs": round(v["gbs"], 1) if v["gbs"] else None}
                     for k, v in kernels.items()},
@@ -377,16 +384,25 @@ def main():
         log("FP64 oracle check: rel-L2 %.3e (%.0fs)" % (err, time.time() - tc))
         del o
     if not args.no_cpu:
+        # median of 5 single-iteration CG runs after one warm-up run (the loop
+        # timer covers the iteration only, not the RHS formation)
         o = orc.OracleReco(grid, samp, C, nthreads=threads)
         o.set_trajectory(traj)
         o.calculate_weights(data)
         o.calculate_kernel_ft()
         o.set_maps(maps)
-        _, _, itc, loop_s = o.cg_sense(data, 0.0, 0, prec=32, timed=True)
-        out["cpu_baseline"] = {"value": itc / loop_s, "unit": "iterations/s", "cores": threads, "kind": "port",
-                               "sample": "%d CG iteration(s) of the same workload, FP32 restatement of "
-                                         "Reco.cpp:175-553 (coil-parallel OpenMP, own CPU FFT)" % itc}
-        log("CPU baseline: %.3f it/s on %d threads" % (itc / loop_s, threads))
+        times = []
+        for rep in range(6):
+            _, _, itc, loop_s = o.cg_sense(data, 0.0, 0, prec=32, timed=True)
+            if rep > 0:
+                times.append(loop_s / itc)
+        med = float(np.median(times))
+        out["cpu_baseline"] = {"value": 1.0 / med, "unit": "iterations/s", "cores": threads, "kind": "port",
+                               "sample": "median of 5 timed CG iterations (after 1 warm-up) of the same workload, "
+                                         "FP32 restatement of Reco.cpp:175-553 (coil-parallel OpenMP, own CPU FFT; "
+                                         "FFTW is not available, so this understates the reference's FFT speed)"}
+        log("CPU baseline: %.3f it/s on %d threads (median of %s s)" % (1.0 / med, threads,
+                                                                        ", ".join("%.2f" % t for t in times)))
     print(json.dumps(out), flush=True)
 
 
