@@ -320,6 +320,35 @@ struct LastStageMap {
     __device__ static __forceinline__ int line(int p, int i) { return ((int)threadIdx.x + p * NT) / NCOL + i * LS; }
 };
 
+// Last stage into registers: out[p][i] is the output of slot (p, i) (line
+// and column from LastStageMap), left to the caller to store.
+template <int L, int DIR, int NCOL, int STRIDE, int NT, int PER, int R>
+__device__ __forceinline__ void fft_last_stage_regs(const float2* lds, const float2* tw, float2 (&out)[PER][R]) {
+    constexpr int NS = plan_nstages(L);
+    constexpr int LS = plan_ls(L, NS - 1);
+    constexpr int NB = NCOL * (L / R);
+    static_assert(R == plan_radix(L, NS - 1) && PER == (NB + NT - 1) / NT, "register tile shape");
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int b = tid + p * NT;
+        if ((NB % NT) == 0 || b < NB) {
+            const int col = b % NCOL, j = b / NCOL;
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                float2 x = lds[(j + i * LS) * STRIDE + col];
+                if (LS > 1 && i > 0) {
+                    float2 t = tw[j * i];
+                    if (DIR > 0) t.y = -t.y;
+                    x = cmul(x, t);
+                }
+                out[p][i] = x;
+            }
+            Dft<R, DIR>::run(out[p]);
+        }
+    }
+}
+
 // Line transform of an [L x NCOL] tile already in LDS (and visible to the
 // block): every stage but the last through LDS, the last into st.
 template <int L, int DIR, int NCOL, int STRIDE, bool LDS_STORE, int NT, class Store>
@@ -611,15 +640,38 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     TileRegs<L, W, NT, float2> next;
     next.load(ld(0));
     __syncthreads();   // pline / kline / twl visible
+    // The outputs of an even chunk are held in registers and stored together
+    // with the next chunk's, so that both halves of a cell's 16-coil line are
+    // written back to back.
+    using LM = LastStageMap<L, W, NT>;
+    constexpr int NS = plan_nstages(L);
+    float2 hold[LM::PER][LM::R];
     for (int c0 = 0; c0 < C; c0 += W) {
         cur = c0;
         next.template to_lds<W>(lds, cv);
         __syncthreads();
         if (c0 + W < C) next.load(ld(c0 + W));
-        fft_lds_tile<L, -1, W, W, false, NT>(lds, twl, [&](int l, int c, float2 v, int, int) {
-            const int coil = c0 + c;
-            if (coil < C) g[l * C + coil] = alt_sign(v, l);
-        });
+        fft_mid_stages<L, 0, NS - 1, 1, -1, W, W, NT>(lds, twl);
+        float2 out[LM::PER][LM::R];
+        fft_last_stage_regs<L, -1, W, W, NT>(lds, twl, out);
+        const bool keep = ((c0 / W) % 2 == 0) && (c0 + W < C);
+        const bool pair = (c0 / W) % 2 == 1;
+#pragma unroll
+        for (int p = 0; p < LM::PER; ++p) {
+            if (!LM::valid(p)) continue;
+            const int c = LM::col(p);
+#pragma unroll
+            for (int i = 0; i < LM::R; ++i) {
+                const int l = LM::line(p, i);
+                const float2 o = alt_sign(out[p][i], l);
+                if (keep) {
+                    hold[p][i] = o;
+                } else {
+                    if (pair) g[l * C + c0 - W + c] = hold[p][i];
+                    if (c0 + c < C) g[l * C + c0 + c] = o;
+                }
+            }
+        }
         if (c0 + W < C) __syncthreads();
     }
 }
