@@ -500,9 +500,9 @@ __device__ __forceinline__ float2 alt_sign(float2 v, int l) {
 // Strided pass (y or x axis): block = one outer index x CPB consecutive
 // chunks of W columns; lines of length L with stride a.ls.  The first-stage
 // inputs of the next chunk are fetched while the current chunk is transformed.
-#ifndef CGS_PLAIN_CPB
-#define CGS_PLAIN_CPB 2
-#endif
+// Chunks per block: 4 for passes whose lines stride a megabyte or more (x),
+// 3 otherwise (measured best on the 320x320x224 grid).
+constexpr int plain_cpb(bool far) { return far ? 4 : 3; }
 // Element at a 32-bit byte offset (or a 64-bit element offset) from a
 // block-uniform base: the 32-bit form lets the load use a scalar base address
 // plus one offset register.
@@ -833,19 +833,28 @@ int zpass_w(int L, int C) {
     return 8;
 }
 
-template <int L, int DIR>
-static void plain_launch(const PlainArgs& a, dim3 grid, hipStream_t s) {
+static inline bool plain_far(const PlainArgs& a) { return a.ls * (long long)sizeof(float2) >= (1LL << 20); }
+
+template <int L, int DIR, int CPB>
+static void plain_launch_cpb(const PlainArgs& a, dim3 grid, hipStream_t s) {
     constexpr int W = plain_w(L);
     // largest byte offset from a chunk's first column
     const long long span = ((long long)(L - 1) * a.ls + a.ncols) * (long long)sizeof(float2);
-    if (span < (1LL << 32)) k_fft_plain<L, W, DIR, CGS_PLAIN_CPB, uint32_t><<<grid, CGS_THREADS, 0, s>>>(a);
-    else k_fft_plain<L, W, DIR, CGS_PLAIN_CPB, long long><<<grid, CGS_THREADS, 0, s>>>(a);
+    if (span < (1LL << 32)) k_fft_plain<L, W, DIR, CPB, uint32_t><<<grid, CGS_THREADS, 0, s>>>(a);
+    else k_fft_plain<L, W, DIR, CPB, long long><<<grid, CGS_THREADS, 0, s>>>(a);
+}
+
+template <int L, int DIR>
+static void plain_launch(const PlainArgs& a, dim3 grid, hipStream_t s) {
+    if (plain_far(a)) plain_launch_cpb<L, DIR, plain_cpb(true)>(a, grid, s);
+    else plain_launch_cpb<L, DIR, plain_cpb(false)>(a, grid, s);
 }
 
 hipError_t launch_fft_plain(int L, int dir, const PlainArgs& a, long long nouter, hipStream_t s) {
     const int W = plain_w(L);
     const long long nchunk = (a.ncols + W - 1) / W;
-    const long long ngrp = (nchunk + CGS_PLAIN_CPB - 1) / CGS_PLAIN_CPB;
+    const int cpb = plain_cpb(plain_far(a));
+    const long long ngrp = (nchunk + cpb - 1) / cpb;
     const dim3 grid((unsigned)(nouter * ngrp));
     switch (L) {
 #define CGS_CASE(n)                                            \

@@ -171,7 +171,7 @@ def bench_binned(args, cfg, pkg, torch):
         "data": "synthetic (seeded phantom x smooth coil maps + noise, wave-CAIPI, seeded respiratory bins)",
         "config": {"workload": args.workload, "image": list(image), "grid": list(grid), "coils": C, "bins": B,
                    "samples_per_bin": counts, "acceleration": list(cfg["R"]), "cg_iterations": K,
-                   "gridding_tile": list(ctx.tile_config()[1])},
+                   "gridding_tile": list(ctx.tile_config()[1]), "tile_config": int(ctx.tile_config()[0])},
         "plain_path_value": plain_value,
         "batched_over_plain": value / plain_value,
         "voronoi_seconds": voronoi_s,
@@ -354,7 +354,7 @@ def main():
         "data": "synthetic (seeded phantom x smooth coil maps + noise, wave-CAIPI)",
         "config": {"workload": args.workload, "image": list(image), "grid": list(grid), "coils": C,
                    "samples_per_coil": int(m), "acceleration": list(cfg["R"]), "cg_iterations": K,
-                   "gridding_tile": list(ctx.tile_config()[1])},
+                   "gridding_tile": list(ctx.tile_config()[1]), "tile_config": int(ctx.tile_config()[0])},
         "plain_path_value": plain_value,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (dk["gbs"] / HBM_PEAK_GBS) if dk["gbs"] else None,
