@@ -15,6 +15,7 @@
 namespace cgs {
 
 constexpr int kRedThreads = 256;
+constexpr int kScalarThreads = 1024;   // blocks that reduce partial lists
 
 __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
     for (int off = 32; off >= 1; off >>= 1) {
@@ -52,14 +53,36 @@ __global__ __launch_bounds__(kRedThreads) void k_dot_partials(const float2* __re
     }
 }
 
+// Sum of nblk (re, im) partials of one bin by one block.  Each thread keeps
+// eight independent accumulators (eight loads in flight; the partial lists
+// reach 10^5 entries), combined in a fixed order, so the result does not
+// depend on timing.
 __device__ void reduce_parts(const double* part, int nblk, int bin, double* out, double* red) {
-    double a = 0.0, b = 0.0;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-        a += part[((long long)bin * nblk + i) * 2];
-        b += part[((long long)bin * nblk + i) * 2 + 1];
+    constexpr int U = 8;
+    double a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = b[u] = 0.0;
+    const double2* pp = (const double2*)part + (long long)bin * nblk;
+    const int step = blockDim.x * U;
+    int i0 = threadIdx.x;
+    for (; i0 + (U - 1) * (int)blockDim.x < nblk; i0 += step) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double2 v = pp[i0 + u * blockDim.x];
+            a[u] += v.x;
+            b[u] += v.y;
+        }
     }
-    block_sum2(a, b, red);
-    if (threadIdx.x == 0) { out[0] = a; out[1] = b; }
+    for (int i = i0; i < nblk; i += blockDim.x) {
+        const double2 v = pp[i];
+        a[0] += v.x;
+        b[0] += v.y;
+    }
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) { sa += a[u]; sb += b[u]; }
+    block_sum2(sa, sb, red);
+    if (threadIdx.x == 0) { out[0] = sa; out[1] = sb; }
 }
 
 __device__ __forceinline__ void cdiv(const double* x, const double* y, double* r) {
@@ -69,8 +92,9 @@ __device__ __forceinline__ void cdiv(const double* x, const double* y, double* r
 }
 
 // rr = <r0, r0> of the right-hand side; reset the per-bin state.
-__global__ void k_cg_init(CgScalars* sc, const double* part, int nblk, double thresh, int* done) {
-    __shared__ double red[2 * (kRedThreads / 64)];
+__global__ __launch_bounds__(kScalarThreads) void k_cg_init(CgScalars* sc, const double* part, int nblk, double thresh,
+                                                          int* done) {
+    __shared__ double red[2 * (kScalarThreads / 64)];
     const int bin = blockIdx.x;
     double rr[2];
     reduce_parts(part, nblk, bin, rr, red);
@@ -87,8 +111,9 @@ __global__ void k_cg_init(CgScalars* sc, const double* part, int nblk, double th
 }
 
 // pq = <p, q>; NaN/Inf check of q[0] (Reco.cpp:264-268); alpha = rr / pq.
-__global__ void k_cg_alpha(CgScalars* sc, const double* part, int nblk, const float2* q, long long ng, int* done) {
-    __shared__ double red[2 * (kRedThreads / 64)];
+__global__ __launch_bounds__(kScalarThreads) void k_cg_alpha(CgScalars* sc, const double* part, int nblk, const float2* q,
+                                                           long long ng, int* done) {
+    __shared__ double red[2 * (kScalarThreads / 64)];
     const int bin = blockIdx.x;
     CgScalars& s = sc[bin];
     if (s.done) return;
@@ -237,13 +262,13 @@ hipError_t launch_dot_partials(const float2* a, const float2* b, long long n, in
 
 hipError_t launch_cg_init(CgScalars* sc, const double* part, int nblk, int nbins, double thresh, int* done,
                           hipStream_t s) {
-    k_cg_init<<<nbins, kRedThreads, 0, s>>>(sc, part, nblk, thresh, done);
+    k_cg_init<<<nbins, kScalarThreads, 0, s>>>(sc, part, nblk, thresh, done);
     return hipGetLastError();
 }
 
 hipError_t launch_cg_alpha(CgScalars* sc, const double* part, int nblk, int nbins, const float2* q, long long ng,
                            int* done, hipStream_t s) {
-    k_cg_alpha<<<nbins, kRedThreads, 0, s>>>(sc, part, nblk, q, ng, done);
+    k_cg_alpha<<<nbins, kScalarThreads, 0, s>>>(sc, part, nblk, q, ng, done);
     return hipGetLastError();
 }
 
