@@ -73,6 +73,8 @@ def lib():
         "cgs_apply_normal": (I, [P, P, P]),
         "cgs_cg_sense_images": (I, [P, P, D, I, P, P, P]),
         "cgs_set_num_bins": (I, [P, I]),
+        "cgs_set_bin_group": (I, [P, I]),
+        "cgs_get_bin_group": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_set_bin_data": (I, [P, I, P, P]),
         "cgs_set_bin_samples": (I, [P, I, LL, P, P, P]),
         "cgs_cg_prepare": (I, [P, D, I]),
@@ -250,6 +252,16 @@ class Context:
     def set_num_bins(self, n):
         self._ck(self._L.cgs_set_num_bins(self.h, int(n)))
         self.nbins = int(n)
+
+    def set_bin_group(self, max_bins):
+        """Cap on the bins one batched launch covers (0: device memory decides)."""
+        self._ck(self._L.cgs_set_bin_group(self.h, int(max_bins)))
+
+    def bin_group(self):
+        """Bins per batched launch chosen by the last cg_prepare."""
+        g = ctypes.c_int()
+        self._ck(self._L.cgs_get_bin_group(self.h, ctypes.byref(g)))
+        return g.value
 
     def set_bin_data(self, b, data, dcf=None):
         d = _c64(data, self.ncoil * self.ns)

@@ -99,12 +99,14 @@ struct cgs_ctx {
     int max_iter = 15;
     bool cg_ready = false;
     int group = 1;                 // bins per batched launch
+    int group_cap = 0;             // caller's limit on bins per launch (0: memory only)
 
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     double g_thresh = 0.0;         // loop-control values the graph was captured with
     int g_max_iter = -1;
     double* g_hist = nullptr;
+    int g_group = 0;
 
     bool prof = false;
     std::vector<ProfEntry> pending;
@@ -149,6 +151,13 @@ int fail(cgs_ctx* h, int code, const std::string& msg) {
 
 int fail_hip(cgs_ctx* h, const char* what, hipError_t e) {
     return fail(h, CGS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// The captured batched iteration bakes in buffer addresses and the group
+// split; it is dropped whenever one of them changes.
+void drop_graph(cgs_ctx* h) {
+    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
 }
 
 #define CK(call)                                                   \
@@ -335,8 +344,7 @@ void free_packed(cgs_ctx* h) {
     dfree(h, h->p_yw, mt * h->C);
     h->p_alias = false;
     h->packed = false;
-    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
-    if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+    drop_graph(h);
 }
 
 // Binning + stable tile sort + KB tables of the active samples among traj[ns]
@@ -478,7 +486,7 @@ int ensure_grid(cgs_ctx* h, int nb) {
     h->grid_cap = 0;
     CKS(dalloc(h, &h->d_grid, (size_t)nb * h->ng * h->C));
     h->grid_cap = nb;
-    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    drop_graph(h);
     return CGS_OK;
 }
 
@@ -501,7 +509,7 @@ int ensure_vectors(cgs_ctx* h) {
     CKS(dalloc(h, &h->d_done, nb));
     CK(hipMemset(h->d_done, 0, sizeof(int) * nb));
     h->vec_bins = nb;
-    if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+    drop_graph(h);
     return CGS_OK;
 }
 
@@ -653,6 +661,7 @@ int choose_group(cgs_ctx* h) {
     const double have = (double)fr + (double)h->grid_cap * per - 2.0e9;   // keep 2 GB headroom
     int g = (int)std::floor(have / per);
     g = std::max(1, std::min(g, h->nbins));
+    if (h->group_cap > 0) g = std::min(g, h->group_cap);
     // equal groups
     const int ngroups = (h->nbins + g - 1) / g;
     h->group = (h->nbins + ngroups - 1) / ngroups;
@@ -942,6 +951,19 @@ int cgs_set_num_bins(cgs_ctx* h, int nbins) {
     return CGS_OK;
 }
 
+int cgs_set_bin_group(cgs_ctx* h, int max_bins) {
+    if (!h || max_bins < 0) return CGS_ERR_ARG;
+    h->group_cap = max_bins;
+    h->cg_ready = false;
+    return CGS_OK;
+}
+
+int cgs_get_bin_group(const cgs_ctx* h, int* bins_per_launch) {
+    if (!h || !bins_per_launch) return CGS_ERR_ARG;
+    *bins_per_launch = h->group;
+    return CGS_OK;
+}
+
 int cgs_set_bin_data(cgs_ctx* h, int bin, const float* data, const float* dcf) {
     if (!h || !data || bin < 0 || bin >= h->nbins) return CGS_ERR_ARG;
     float2* dd = nullptr;
@@ -1040,11 +1062,9 @@ int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter) {
     h->cg_ready = true;
     // The captured iteration bakes in the loop-control scalars and the history
     // buffer; keep it while those stay the same.
-    if (h->gexec && (h->g_thresh != thresh || h->g_max_iter != max_iter || h->g_hist != h->d_hist)) {
-        (void)hipGraphExecDestroy(h->gexec);
-        h->gexec = nullptr;
-        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
-    }
+    if (h->gexec && (h->g_thresh != thresh || h->g_max_iter != max_iter || h->g_hist != h->d_hist ||
+                     h->g_group != h->group))
+        drop_graph(h);
     return CGS_OK;
 }
 
@@ -1093,6 +1113,7 @@ int cgs_cg_iterate(cgs_ctx* h, int mode, int n_iter) {
                 h->g_thresh = h->thresh;
                 h->g_max_iter = h->max_iter;
                 h->g_hist = h->d_hist;
+                h->g_group = h->group;
             }
             for (int it = 0; it < n_iter; ++it) CK(hipGraphLaunch(h->gexec, h->st));
         }

@@ -35,15 +35,18 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(name, ng, C, m, vs):
-    """HBM bytes one launch must move at minimum (complex64 = 8 B)."""
+def algorithmic_bytes(name, ng, C, m, vs, touched=None):
+    """HBM bytes one launch must move at minimum (complex64 = 8 B).  touched:
+    grid cells within the KB window of at least one sample (gridding reads or
+    writes only those; the whole grid when not given)."""
+    nt = ng if touched is None else touched
     per = {
         "fft_z_fwd": (16 * C + 12) * ng,             # p, S_c, kft in; coil grids out
         "fft_y_fwd": 16 * C * ng, "fft_x_fwd": 16 * C * ng,
         "fft_x_inv": 16 * C * ng, "fft_y_inv": 16 * C * ng,
         "fft_z_inv": (16 * C + 20) * ng,             # grids, S_c, kft, p in; q out
-        "degrid": 8 * C * ng + m * (16 + 8 * vs + 4 + 8 * C),
-        "grid": 8 * C * ng + m * (16 + 8 * vs + 8 * C),
+        "degrid": 8 * C * nt + m * (16 + 8 * vs + 4 + 8 * C),
+        "grid": 8 * C * nt + m * (16 + 8 * vs + 8 * C),
         "cg_update_xr": 48 * ng,
         "cg_update_p": 24 * ng,
         "cg_alpha": 0, "cg_beta": 0,
@@ -51,14 +54,58 @@ def algorithmic_bytes(name, ng, C, m, vs):
     return per.get(name, 0)
 
 
+def touched_cells(k, grid, kw):
+    """Grid cells the gridding kernels touch for samples k (m x 3 in [0, 1]):
+    the kw cells per axis around each sample's nearest cell, (int)((N-1)k+0.5)
+    (Reco.cpp:705-712), clamped to the grid."""
+    half = kw // 2
+    near = [np.floor((n - 1) * k[:, a] + 0.5).astype(np.int64) for a, n in enumerate(grid)]
+    mark = np.zeros(int(np.prod(grid)), bool)
+    for ox in range(-half, half + 1):
+        ix = np.clip(near[0] + ox, 0, grid[0] - 1) * grid[1]
+        for oy in range(-half, half + 1):
+            ixy = (ix + np.clip(near[1] + oy, 0, grid[1] - 1)) * grid[2]
+            for oz in range(-half, half + 1):
+                mark[ixy + np.clip(near[2] + oz, 0, grid[2] - 1)] = True
+    return int(mark.sum())
+
+
+def stage_rates(rep, n_iter, bytes_per_iter):
+    """Per-stage launch count, average launch ms and algorithmic GB/s from a
+    profile report of n_iter iterations (bytes_per_iter: stage -> bytes one
+    iteration moves at minimum over all bins)."""
+    out = {}
+    for name, (cnt, ms) in rep.items():
+        b = bytes_per_iter.get(name, 0) * n_iter
+        out[name] = dict(count=cnt, avg_ms=ms / cnt, bytes=b / cnt if b else 0,
+                         gbs=(b / (ms * 1e-3) / 1e9) if b else None)
+    return out
+
+
+def roofline_of(kernels, workload):
+    """Roofline object of the dominant stage (largest total time)."""
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count"])
+    prof_mod = importlib.import_module("4d_lung_amd.profiling")
+    traffic_src = os.path.join("profiles", "traffic_%s.txt" % workload)
+    traffic = prof_mod.read_traffic(os.path.join(ROOT, traffic_src))
+    dk = kernels[dom]
+    return {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (dk["gbs"] / HBM_PEAK_GBS) if dk["gbs"] else None,
+            "traffic": traffic[dom][0] + traffic[dom][1] if dom in traffic else None,
+            "traffic_source": traffic_src if dom in traffic else None,
+            "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"]}
+
+
 def bench_binned(args, cfg, pkg, torch):
     """Respiratory-resolved workload: every bin is its own CG-SENSE solve with
     its own Voronoi density weights; all bins share trajectory, maps and KB
-    tables.  A batched step (all bins per launch, device-side loop control,
-    captured in a hipGraph) counts one iteration per bin; the plain path runs
-    the bins one after another with a host read of the residual per
+    tables.  Bins are held in compact form (acquired sample indices, values and
+    weights).  A batched step (groups of bins per launch, device-side loop
+    control, captured in a hipGraph) counts one iteration per bin; the plain
+    path runs the bins one after another with a host read of the residual per
     iteration, as the reference's per-state loop does (CGSense.cpp:200-282)."""
     synth = pkg.synth
+    tune = importlib.import_module("4d_lung_amd.autotune")
     grid, samp, image, C, B = cfg["grid"], cfg["samp"], cfg["image"], cfg["coils"], cfg["bins"]
     ng, ns = int(np.prod(grid)), int(np.prod(samp))
     K, W = args.steps, args.warmup
@@ -69,6 +116,7 @@ def bench_binned(args, cfg, pkg, torch):
     smask = synth.sample_mask_from_lines(lines, samp[0])
     maps = synth.coil_maps(grid, C, cfg["seed"] + 1)
     rho = synth.phantom(grid, image, cfg["seed"] + 2).reshape(-1)
+    log("synthetic image and maps %.1fs (%d lines in %d bins)" % (time.time() - t0, int(lines.sum()), B))
     ctx = pkg.Context(grid, samp, C)
     ctx.set_trajectory(traj)
     ctx.calculate_kernel_ft()
@@ -76,55 +124,49 @@ def bench_binned(args, cfg, pkg, torch):
     mask_data = np.zeros((1, ns), np.complex64)
     mask_data[0, smask] = 1
     ctx.exclude_missing_data(mask_data)
+    del mask_data
     img = maps * rho[None, :]
     y = ctx.nufft_fwd(img)
     del img
     data = synth.add_noise(y, smask, 1e-3, cfg["seed"] + 3)
     del y
-    log("synthetic inputs %.1fs (%d lines in %d bins)" % (time.time() - t0, int(lines.sum()), B))
-    # per-bin data (zero-filled, other bins' lines zero) and Voronoi weights
-    bin_data, bin_dcf, counts = [], [], []
+    log("k-space synthesized %.1fs" % (time.time() - t0))
+    # per-bin compact samples and (per the workload) Voronoi weights
+    bins = []
     tv = time.time()
+    voronoi = cfg.get("dcf", "voronoi") == "voronoi"
     for b in range(B):
-        sel = synth.sample_mask_from_lines(labels == b, samp[0])
-        d = np.where(sel[None, :], data, 0).astype(np.complex64)
-        act = np.flatnonzero(sel)
-        w = np.zeros(ns, np.float32)
-        w[act] = pkg.cgs.voronoi_density(traj[act])
-        bin_data.append(d)
-        bin_dcf.append(w)
-        counts.append(len(act))
-    voronoi_s = time.time() - tv
-    log("per-bin Voronoi density weights: %.1fs for %d samples (bins of %d..%d)"
-        % (voronoi_s, sum(counts), min(counts), max(counts)))
+        idx = np.flatnonzero(synth.sample_mask_from_lines(labels == b, samp[0])).astype(np.int32)
+        w = pkg.cgs.voronoi_density(traj[idx]) if voronoi else None
+        bins.append((idx, np.ascontiguousarray(data[:, idx]), w))
+        if voronoi:
+            log("bin %d: %d samples, Voronoi weights after %.1fs" % (b, idx.size, time.time() - tv))
+    counts = [int(bb[0].size) for bb in bins]
+    voronoi_s = time.time() - tv if voronoi else None
+    if voronoi:
+        log("per-bin Voronoi density weights: %.1fs for %d samples (bins of %d..%d)"
+            % (voronoi_s, sum(counts), min(counts), max(counts)))
+    else:
+        log("bins of %d..%d samples, no density weights" % (min(counts), max(counts)))
     del data
     ctx.set_num_bins(B)
 
     def load_bins():
-        for b in range(B):
-            ctx.set_bin_data(b, bin_data[b], bin_dcf[b])
+        for b, (idx, yb, w) in enumerate(bins):
+            ctx.set_bin_samples(b, idx, yb, w)
 
-    # autotune the gridding tile on one batched iteration of all bins
+    # autotune: gridding tile on one batched iteration, then bins per launch
     if args.tile < 0:
-        best = None
-        for t in range(pkg.cgs.lib().cgs_num_tile_configs()):
-            ctx.set_tile_config(t)
-            load_bins()
-            ctx.cg_prepare(0.0, 2)
-            ctx.profile(True)
-            ctx.profile_reset()
-            ctx.cg_iterate(pkg.SOLVE_BATCHED, 1)
-            rep = ctx.profile_report()
-            ctx.profile(False)
-            ms = rep["grid"][1] + rep["degrid"][1]
-            log("tile config %d %s: grid+degrid %.3f ms per batched iteration" % (t, ctx.tile_config()[1], ms))
-            if best is None or ms < best[1]:
-                best = (t, ms)
-        tcfg = best[0]
+        tcfg, _ = tune.tune_tile(ctx, load_bins, log)
     else:
         tcfg = args.tile
-    ctx.set_tile_config(tcfg)
-    load_bins()
+        ctx.set_tile_config(tcfg)
+        load_bins()
+    if args.bin_group < 0:
+        group, _ = tune.tune_group(ctx, 2, log)
+    else:
+        group = args.bin_group
+        ctx.set_bin_group(group)
 
     ctx.cg_prepare(0.0, K - 1)
     ctx.cg_iterate(pkg.SOLVE_BATCHED, max(1, W))
@@ -134,10 +176,11 @@ def bench_binned(args, cfg, pkg, torch):
     ctx.cg_iterate(pkg.SOLVE_BATCHED, K)
     torch.cuda.synchronize()
     secs = time.perf_counter() - ts
-    x, res, its = ctx.cg_results()
+    x, res, its = ctx.cg_results(want_x=(args.check_iters is None or args.check_iters == K))
     assert all(int(i) == K for i in its), its
     value = B * K / secs
-    log("batched: %d iterations of %d bins in %.3f s: %.1f bin-iterations/s" % (K, B, secs, value))
+    log("batched: %d iterations of %d bins (%d per launch) in %.3f s: %.2f bin-iterations/s"
+        % (K, B, ctx.bin_group(), secs, value))
     ctx.cg_prepare(0.0, K - 1)
     torch.cuda.synchronize()
     tp = time.perf_counter()
@@ -145,19 +188,29 @@ def bench_binned(args, cfg, pkg, torch):
     torch.cuda.synchronize()
     plain_secs = time.perf_counter() - tp
     plain_value = B * K / plain_secs
-    log("plain (bin after bin, host residual read per iteration): %.1f bin-iterations/s, batched/plain %.2fx"
+    log("plain (bin after bin, host residual read per iteration): %.2f bin-iterations/s, batched/plain %.2fx"
         % (plain_value, value / plain_value))
+    # per-stage HIP-event timing over 2 batched iterations
+    n_prof = 2
     ctx.cg_prepare(0.0, K - 1)
     ctx.profile(True)
     ctx.profile_reset()
-    ctx.cg_iterate(pkg.SOLVE_BATCHED, 2)
+    ctx.cg_iterate(pkg.SOLVE_BATCHED, n_prof)
     rep = ctx.profile_report()
     ctx.profile(False)
-    kernels = {name: round(ms / cnt, 4) for name, (cnt, ms) in rep.items()}
-    for name in sorted(kernels, key=lambda k: -kernels[k]):
-        log("  %-14s %8.3f ms per batched launch" % (name, kernels[name]))
+    kw = ctx.kernel_info()[0]
+    vs = (3 * kw + 1) & ~1
+    touched = [touched_cells(traj[idx], grid, kw) for idx, _, _ in bins]
+    per_iter = {name: sum(algorithmic_bytes(name, ng, C, m, vs, t) for m, t in zip(counts, touched))
+                for name in rep}
+    kernels = stage_rates(rep, n_prof, per_iter)
+    for name in sorted(kernels, key=lambda k: -kernels[k]["avg_ms"] * kernels[k]["count"]):
+        kk = kernels[name]
+        log("  %-14s %8.3f ms per launch (%d launches)  %s" % (
+            name, kk["avg_ms"], kk["count"], "%.0f GB/s" % kk["gbs"] if kk["gbs"] else ""))
+    total_bytes = sum(per_iter.values())
     out = {
-        "metric": "CG-SENSE bin-iterations/sec (E^H E applies/s) at \u22641e-4 rel-L2, 1 MI355X, batched vs plain",
+        "metric": "CG-SENSE bin-iterations/sec (E^H E applies/s) at ≤1e-4 rel-L2, 1 MI355X, batched vs plain",
         "value": value,
         "unit": "bin-iterations/s",
         "n_gpus": 1,
@@ -171,46 +224,102 @@ def bench_binned(args, cfg, pkg, torch):
         "data": "synthetic (seeded phantom x smooth coil maps + noise, wave-CAIPI, seeded respiratory bins)",
         "config": {"workload": args.workload, "image": list(image), "grid": list(grid), "coils": C, "bins": B,
                    "samples_per_bin": counts, "acceleration": list(cfg["R"]), "cg_iterations": K,
-                   "gridding_tile": list(ctx.tile_config()[1]), "tile_config": int(ctx.tile_config()[0])},
+                   "gridding_tile": list(ctx.tile_config()[1]), "tile_config": int(ctx.tile_config()[0]),
+                   "bins_per_launch": ctx.bin_group(), "density": "voronoi per bin" if voronoi else "none"},
         "plain_path_value": plain_value,
         "batched_over_plain": value / plain_value,
+        "roofline": roofline_of(kernels, args.workload),
+        "algorithmic_gbs_per_iteration": total_bytes * value / B / 1e9,
         "voronoi_seconds": voronoi_s,
-        "kernels_ms_per_batched_launch": kernels,
+        "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_iteration": v["count"] // n_prof,
+                        "gbs": round(v["gbs"], 1) if v["gbs"] else None} for k, v in kernels.items()},
     }
+    ci = K if args.check_iters is None else args.check_iters
+    check_bins = list(range(B)) if args.check_all_bins else [int(v) for v in args.check_bins.split(",") if v]
+    if not args.no_check and ci != K:
+        # the oracle's FP64 solve of the full iteration count does not fit the
+        # run: both sides solve the checked bins for ci iterations instead
+        ctx.cg_prepare(0.0, ci - 1)
+        ctx.cg_iterate(pkg.SOLVE_BATCHED, ci)
+        x, _, its = ctx.cg_results()
+        assert all(int(i) == ci for i in its), its
     del ctx
-    if not args.no_check:
-        from oracle import oracle as orc
+    from oracle import oracle as orc
 
-        orc.set_verbose(1)
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    orc.set_verbose(1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+    def oracle_for(b):
+        idx, yb, w = bins[b]
+        d = np.zeros((C, ns), np.complex64)
+        d[:, idx] = yb
+        wf = None
+        if w is not None:
+            wf = np.zeros(ns, np.float32)
+            wf[idx] = w
+        o = orc.OracleReco(grid, samp, C, nthreads=threads)
+        o.set_trajectory(traj)
+        o.calculate_weights(d)
+        o.calculate_kernel_ft()
+        o.set_maps(maps)
+        o.set_density(wf)
+        return o, d
+
+    if not args.no_check:
         errs = {}
-        for b in ([0] if not args.check_all_bins else range(B)):
+        for b in check_bins:
             tc = time.time()
-            o = orc.OracleReco(grid, samp, C, nthreads=threads)
-            o.set_trajectory(traj)
-            o.calculate_weights(bin_data[b])
-            o.calculate_kernel_ft()
-            o.set_maps(maps)
-            o.set_density(bin_dcf[b])
-            xr, _, _ = o.cg_sense(bin_data[b], 0.0, K - 1, prec=64)
+            o, d = oracle_for(b)
+            xr, _, _ = o.cg_sense(d, 0.0, ci - 1, prec=64)
             errs[b] = float(np.linalg.norm(x[b].ravel() - xr.ravel()) / np.linalg.norm(xr.ravel()))
-            log("FP64 oracle check bin %d: rel-L2 %.3e (%.0fs)" % (b, errs[b], time.time() - tc))
-            del o
+            log("FP64 oracle check bin %d after %d iterations: rel-L2 %.3e (%.0fs)" % (b, ci, errs[b], time.time() - tc))
+            del o, d, xr
         out["rel_l2_vs_fp64_oracle_per_bin"] = errs
+        out["check_iterations"] = ci
         out["within_tolerance"] = bool(max(errs.values()) <= 1e-4)
+    del x
+    if not args.no_cpu:
+        # bin 0 on the CPU: timed CG iterations of the FP32 restatement (the
+        # loop timer covers the iteration, not the right-hand side); median of
+        # the timed runs after one warm-up run
+        o, d = oracle_for(0)
+        times = []
+        reps = 6
+        for r in range(reps):
+            _, _, itc, loop_s = o.cg_sense(d, 0.0, 0, prec=32, timed=True)
+            if r == 0 and loop_s / itc > 5.0:
+                reps = 3
+            if r > 0:
+                times.append(loop_s / itc)
+            if r + 1 >= reps:
+                break
+        med = float(np.median(times))
+        out["cpu_baseline"] = {"value": 1.0 / med, "unit": "bin-iterations/s", "cores": threads, "kind": "port",
+                               "sample": "median of %d timed CG iterations (after 1 warm-up) of bin 0 of the same "
+                                         "workload, FP32 restatement of Reco.cpp:175-553 (coil-parallel OpenMP, own "
+                                         "CPU FFT; FFTW is not available, so this understates the reference's FFT "
+                                         "speed)" % len(times)}
+        log("CPU baseline: %.4f bin-it/s on %d threads (median of %s s)" % (
+            1.0 / med, threads, ", ".join("%.2f" % t for t in times)))
+        del o, d
     return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None, help="timed CG iterations (default: the workload's own)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="single_160x160x112")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle accuracy check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tile", type=int, default=-1, help="gridding tile config (-1: autotune)")
     ap.add_argument("--check-all-bins", action="store_true", help="binned workloads: oracle check of every bin")
+    ap.add_argument("--check-bins", default="0", help="binned workloads: bins checked against the oracle")
+    ap.add_argument("--check-iters", type=int, default=None,
+                    help="binned workloads: iterations of the oracle check (default: the workload's own, else "
+                         "--steps); with fewer than --steps the checked bins are solved again for that many")
+    ap.add_argument("--bin-group", type=int, default=-1, help="binned workloads: bins per launch (-1: autotune)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (the reference kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
@@ -234,6 +343,10 @@ def main():
     pkg = importlib.import_module("4d_lung_amd")
     synth = pkg.synth
     cfg = synth.CONFIGS[args.workload]
+    if args.steps is None:
+        args.steps = cfg["iters"]
+    if args.check_iters is None:
+        args.check_iters = cfg.get("check_iters")
     if cfg["bins"] != 1:
         print(json.dumps(bench_binned(args, cfg, pkg, torch)), flush=True)
         return
@@ -324,9 +437,10 @@ def main():
     rep = ctx.profile_report()
     ctx.profile(False)
     kernels = {}
+    touched = touched_cells(traj[np.flatnonzero(data[0] != 0)], grid, kw)
     for name, (cnt, ms) in rep.items():
         avg = ms / cnt
-        b = algorithmic_bytes(name, ng, C, m, vs)
+        b = algorithmic_bytes(name, ng, C, m, vs, touched)
         kernels[name] = dict(count=cnt, avg_ms=avg, gbs=(b / (avg * 1e-3) / 1e9) if b else None, bytes=b)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count"])
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs of

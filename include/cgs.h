@@ -100,6 +100,13 @@ int cgs_set_num_bins(cgs_ctx* h, int nbins);
  * CGSense.cpp:200-226): active samples = coil-0 value != 0; dcf may be NULL
  * (then the context's density weights, if any, are used). */
 int cgs_set_bin_data(cgs_ctx* h, int bin, const float* data, const float* dcf);
+/* Largest number of bins one batched launch covers (0, the default: as many
+ * as the device memory holds coil grids for; the bins are split into equal
+ * groups).  Takes effect at the next cgs_cg_prepare.  No reference
+ * counterpart: the reference solves one state at a time (CGSense.cpp:200-282). */
+int cgs_set_bin_group(cgs_ctx* h, int max_bins);
+/* Bins per batched launch chosen by the last cgs_cg_prepare. */
+int cgs_get_bin_group(const cgs_ctx* h, int* bins_per_launch);
 /* Compact form: m acquired samples with original indices idx[m] (any order),
  * values y[c][m] complex and optional weights dcf[m]. */
 int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, const float* y, const float* dcf);

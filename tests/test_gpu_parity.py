@@ -30,7 +30,7 @@ def small_dcf(orc):
 
 
 @pytest.mark.parametrize("shape", [(8, 12, 16), (16, 24, 20), (32, 28, 40), (64, 48, 56), (128, 128, 96),
-                                   (96, 112, 160)])
+                                   (96, 112, 160), (512, 24, 320), (24, 512, 40)])
 def test_fft_matches_oracle(pkg, orc, shape):
     rng = np.random.default_rng(sum(shape))
     a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
@@ -248,3 +248,47 @@ def test_compact_bin_samples_equal_zero_filled(pkg, orc, small):
     ctx.set_bin_samples(0, idx, P["data"][:, idx])
     x2, _, _ = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=5)
     np.testing.assert_array_equal(x1, x2)
+
+
+def test_32_coils_in_bin_groups(pkg, orc):
+    """32 coils (the largest coil count: four coil chunks per z-pass tile, one
+    32-coil tile per gridding thread group) and batched launches split into
+    groups of bins: every split gives the same images bit for bit, and each
+    bin's image matches the oracle."""
+    P = small_problem(orc, ncoil=32, seed=41)
+    o = P["oracle"]
+    nb = 3
+    lines = P["lines"]
+    acq = np.flatnonzero(lines.ravel())
+    labels = np.full(lines.size, -1)
+    labels[acq] = np.arange(acq.size) % nb
+    labels = labels.reshape(lines.shape)
+    rng = np.random.default_rng(42)
+    ctx = gpu_context(pkg, P)
+    ctx.set_num_bins(nb)
+    bins = []
+    for b in range(nb):
+        sm = np.broadcast_to((labels == b)[None], (P["samp"][0],) + lines.shape).reshape(-1)
+        idx = np.flatnonzero(sm & (P["data"][0] != 0))
+        w = rng.uniform(0.5, 1.5, idx.size).astype(np.float32)
+        ctx.set_bin_samples(b, idx, P["data"][:, idx], w)
+        bins.append((idx, w))
+    xs = {}
+    for cap in (0, 1, 2):
+        ctx.set_bin_group(cap)
+        x, _, its = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=6)
+        assert ctx.bin_group() == (nb if cap == 0 else cap)
+        assert list(its) == [7] * nb
+        xs[cap] = x
+    np.testing.assert_array_equal(xs[0], xs[1])
+    np.testing.assert_array_equal(xs[0], xs[2])
+    for b in range(nb):
+        idx, w = bins[b]
+        d = np.zeros_like(P["data"])
+        d[:, idx] = P["data"][:, idx]
+        wf = np.zeros(d.shape[1], np.float32)
+        wf[idx] = w
+        o.calculate_weights(d)
+        o.set_density(wf)
+        xr, _, _ = o.cg_sense(d, thresh=0.0, max_iter=6, prec=64)
+        assert rel_l2(xs[0][b], xr) < CG_TOL
