@@ -44,6 +44,7 @@ struct ProfEntry {
 struct cgs_ctx {
     int N[3];
     long long ng;
+    long long xp = 0, bp = 0;      // coil-grid x-plane and bin pitch (elements)
     int S[3];
     long long ns;
     int C;
@@ -295,11 +296,30 @@ int setup_kb(cgs_ctx* h) {
     return CGS_OK;
 }
 
+// Pitch of the coil grids' x planes (elements).  The strided x pass reads a
+// column segment from every plane: planes a multiple of 2 MiB apart put those
+// reads on few HBM channels (512x512x320 grids of 32 coils, planes 40 MiB
+// apart: x pass at 2.9 TB/s; the same planes 4 KiB further apart: 3.75 TB/s),
+// so such planes are padded by 4 KiB.
+#ifndef CGS_PITCH_MOD
+#define CGS_PITCH_MOD (2LL << 20)
+#endif
+#ifndef CGS_PITCH_PAD
+#define CGS_PITCH_PAD (4LL << 10)
+#endif
+long long coil_grid_pitch(int ny, int nz, int C) {
+    long long p = (long long)ny * nz * C;
+    if ((p * (long long)sizeof(float2)) % CGS_PITCH_MOD == 0) p += CGS_PITCH_PAD / (long long)sizeof(float2);
+    return p;
+}
+
 Geom geom(const cgs_ctx* h) {
     Geom g;
     g.nx = h->N[0]; g.ny = h->N[1]; g.nz = h->N[2];
     g.C = h->C;
     g.ng = h->ng;
+    g.xp = h->xp;
+    g.bp = h->bp;
     return g;
 }
 
@@ -482,9 +502,9 @@ int pack_bins(cgs_ctx* h) {
 // --------------------------------------------------------------- workspace
 int ensure_grid(cgs_ctx* h, int nb) {
     if (h->grid_cap >= nb) return CGS_OK;
-    dfree(h, h->d_grid, (size_t)h->grid_cap * h->ng * h->C);
+    dfree(h, h->d_grid, (size_t)h->grid_cap * h->bp);
     h->grid_cap = 0;
-    CKS(dalloc(h, &h->d_grid, (size_t)nb * h->ng * h->C));
+    CKS(dalloc(h, &h->d_grid, (size_t)nb * h->bp));
     h->grid_cap = nb;
     drop_graph(h);
     return CGS_OK;
@@ -521,20 +541,20 @@ int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const 
     ZArgs z{};
     z.g = h->d_grid; z.tw = h->d_tw[2]; z.p = p; z.maps = p ? h->d_maps : nullptr;
     z.kft = deapod ? h->d_kft : nullptr; z.C = h->C; z.ng = h->ng; z.nxy = (long long)h->N[0] * h->N[1];
-    z.ny = h->N[1]; z.done = done;
+    z.xp = h->xp; z.bp = h->bp; z.ny = h->N[1]; z.done = done;
     {
         Prof t(h, "fft_z_fwd");
         CK(launch_fft_zpre(h->N[2], W, z, (long long)nb * z.nxy, h->st));
     }
     PlainArgs y{};
-    y.g = h->d_grid; y.os = (long long)h->N[1] * h->N[2] * h->C; y.ls = (long long)h->N[2] * h->C;
+    y.g = h->d_grid; y.os = h->xp; y.ls = (long long)h->N[2] * h->C;
     y.ncols = y.ls; y.tw = h->d_tw[1]; y.outer_per_bin = h->N[0]; y.done = done;
     {
         Prof t(h, "fft_y_fwd");
         CK(launch_fft_plain(h->N[1], -1, y, (long long)nb * h->N[0], h->st));
     }
     PlainArgs x{};
-    x.g = h->d_grid; x.os = h->ng * h->C; x.ls = (long long)h->N[1] * h->N[2] * h->C; x.ncols = x.ls;
+    x.g = h->d_grid; x.os = h->bp; x.ls = h->xp; x.ncols = (long long)h->N[1] * h->N[2] * h->C;
     x.tw = h->d_tw[0]; x.outer_per_bin = 1; x.done = done;
     {
         Prof t(h, "fft_x_fwd");
@@ -549,14 +569,14 @@ int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const 
 // are given); otherwise the grids receive the coil images.
 int run_inv_fft(cgs_ctx* h, int nb, float2* q, const float2* p, double* part, const int* done) {
     PlainArgs x{};
-    x.g = h->d_grid; x.os = h->ng * h->C; x.ls = (long long)h->N[1] * h->N[2] * h->C; x.ncols = x.ls;
+    x.g = h->d_grid; x.os = h->bp; x.ls = h->xp; x.ncols = (long long)h->N[1] * h->N[2] * h->C;
     x.tw = h->d_tw[0]; x.outer_per_bin = 1; x.done = done;
     {
         Prof t(h, "fft_x_inv");
         CK(launch_fft_plain(h->N[0], +1, x, nb, h->st));
     }
     PlainArgs y{};
-    y.g = h->d_grid; y.os = (long long)h->N[1] * h->N[2] * h->C; y.ls = (long long)h->N[2] * h->C;
+    y.g = h->d_grid; y.os = h->xp; y.ls = (long long)h->N[2] * h->C;
     y.ncols = y.ls; y.tw = h->d_tw[1]; y.outer_per_bin = h->N[0]; y.done = done;
     {
         Prof t(h, "fft_y_inv");
@@ -568,6 +588,7 @@ int run_inv_fft(cgs_ctx* h, int nb, float2* q, const float2* p, double* part, co
     z.q = q; z.part = part;
     z.scale = (float)(1e-3 / std::sqrt((double)h->ng));
     z.C = h->C; z.ng = h->ng; z.nxy = (long long)h->N[0] * h->N[1]; z.ny = h->N[1]; z.done = done;
+    z.xp = h->xp; z.bp = h->bp;
     bool crop = false;
     for (int a = 0; a < 3; ++a) {
         if (h->imgRange[a] != h->N[a]) crop = true;
@@ -699,6 +720,8 @@ int cgs_create(const int grid[3], const int samp[3], int ncoil, cgs_ctx** out) {
     h->ng = (long long)grid[0] * grid[1] * grid[2];
     h->ns = (long long)samp[0] * samp[1] * samp[2];
     h->C = ncoil;
+    h->xp = coil_grid_pitch(grid[1], grid[2], ncoil);
+    h->bp = h->xp * grid[0];
     h->bins.resize(1);
     if (hipStreamCreate(&h->own_st) != hipSuccess) {
         delete h;
@@ -727,7 +750,7 @@ int cgs_destroy(cgs_ctx* h) {
     dfree(h, h->d_kft, h->ng);
     for (int a = 0; a < 3; ++a) dfree(h, h->d_tw[a], h->N[a]);
     dfree(h, h->d_maps, h->ng * h->C);
-    dfree(h, h->d_grid, (size_t)h->grid_cap * h->ng * h->C);
+    dfree(h, h->d_grid, (size_t)h->grid_cap * h->bp);
     const size_t n = (size_t)h->vec_bins * h->ng;
     dfree(h, h->d_x, n); dfree(h, h->d_r, n); dfree(h, h->d_p, n); dfree(h, h->d_q, n);
     dfree(h, h->d_part, 1); dfree(h, h->d_part2, 1); dfree(h, h->d_sc, 1); dfree(h, h->d_done, 1);
@@ -870,7 +893,7 @@ int cgs_set_maps(cgs_ctx* h, const float* maps) {
     float2* tmp = nullptr;
     CKS(upload_complex(h, &tmp, maps, (size_t)h->C * h->ng));
     if (!h->d_maps) CKS(dalloc(h, &h->d_maps, (size_t)h->C * h->ng));
-    CK(launch_transpose_coils(tmp, h->d_maps, h->ng, h->C, 1, h->st));
+    CK(launch_transpose_coils(tmp, h->d_maps, h->ng, h->C, (long long)h->N[1] * h->N[2], (long long)h->N[1] * h->N[2] * h->C, 1, h->st));
     CK(hipStreamSynchronize(h->st));
     dfree(h, tmp, (size_t)h->C * h->ng);
     h->maps_ready = true;
@@ -884,7 +907,7 @@ int cgs_nufft_fwd(cgs_ctx* h, const float* img, float* data) {
     CKS(ensure_grid(h, 1));
     float2* tmp = nullptr;
     CKS(upload_complex(h, &tmp, img, (size_t)h->C * h->ng));
-    CK(launch_transpose_coils(tmp, h->d_grid, h->ng, h->C, 1, h->st));
+    CK(launch_transpose_coils(tmp, h->d_grid, h->ng, h->C, (long long)h->N[1] * h->N[2], h->xp, 1, h->st));
     CKS(run_fwd_fft(h, 0, 1, nullptr, true, nullptr));
     GridArgs g = grid_args(h, 0, 1);
     g.Y = h->p_yw;
@@ -915,7 +938,7 @@ int cgs_nufft_bwd(cgs_ctx* h, const float* data, float* img) {
     CKS(run_inv_fft(h, 1, nullptr, nullptr, nullptr, nullptr));
     float2* out = nullptr;
     CKS(dalloc(h, &out, (size_t)h->C * h->ng));
-    CK(launch_transpose_coils(h->d_grid, out, h->ng, h->C, 0, h->st));
+    CK(launch_transpose_coils(h->d_grid, out, h->ng, h->C, (long long)h->N[1] * h->N[2], h->xp, 0, h->st));
     CK(hipMemcpyAsync(img, out, sizeof(float2) * h->C * h->ng, hipMemcpyDeviceToHost, h->st));
     CK(hipStreamSynchronize(h->st));
     dfree(h, out, (size_t)h->C * h->ng);
@@ -1170,8 +1193,8 @@ int cgs_cg_sense_images(cgs_ctx* h, const float* img, double thresh, int max_ite
     CK(hipMemsetAsync(h->d_done, 0, sizeof(int), h->st));
     float2* tmp = nullptr;
     CKS(upload_complex(h, &tmp, img, (size_t)h->C * h->ng));
-    CK(launch_transpose_coils(tmp, h->d_grid, h->ng, h->C, 1, h->st));
-    CK(launch_coil_combine(h->d_grid, h->d_maps, h->d_r, h->ng, h->C, h->st));
+    CK(launch_transpose_coils(tmp, h->d_grid, h->ng, h->C, (long long)h->N[1] * h->N[2], h->xp, 1, h->st));
+    CK(launch_coil_combine(h->d_grid, h->d_maps, h->d_r, h->ng, h->C, (long long)h->N[1] * h->N[2], h->xp, h->st));
     CK(launch_copy_rhs(h->d_x, h->d_r, h->d_p, h->d_r, h->ng, 1, h->st));
     CK(launch_dot_partials(h->d_r, h->d_r, h->ng, 1, h->nblk_upd, h->d_part2, h->st));
     CK(launch_cg_init(h->d_sc, h->d_part2, h->nblk_upd, 1, thresh, h->d_done, h->st));

@@ -222,14 +222,16 @@ __global__ void k_fill_zero(float2* a, long long n) {
 }
 
 // q[cell] = sum_c conj(S[cell][c]) G[cell][c] -- the initial image of
-// Reco::cgSenseDynamic (Reco.cpp:183-190) from coil images.
+// Reco::cgSenseDynamic (Reco.cpp:183-190) from coil images (G's x planes xp
+// elements apart, nyz = ny*nz cells per plane).
 __global__ void k_coil_combine(const float2* __restrict__ G, const float2* __restrict__ maps, float2* __restrict__ q,
-                               long long ng, int C) {
+                               long long ng, int C, long long nyz, long long xp) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ng) return;
+    const long long gi = (i / nyz) * xp + (i % nyz) * C;
     float2 a = make_float2(0.0f, 0.0f);
     for (int c = 0; c < C; ++c) {
-        const float2 s = maps[i * C + c], v = G[i * C + c];
+        const float2 s = maps[i * C + c], v = G[gi + c];
         a.x += s.x * v.x + s.y * v.y;
         a.y += s.x * v.y - s.y * v.x;
     }
@@ -244,8 +246,9 @@ __global__ void k_scale_d(float2* a, long long n, double f) {
     a[i] = make_float2((float)((double)v.x * f), (float)((double)v.y * f));
 }
 
-hipError_t launch_coil_combine(const float2* G, const float2* maps, float2* q, long long ng, int C, hipStream_t s) {
-    k_coil_combine<<<(unsigned)((ng + 255) / 256), 256, 0, s>>>(G, maps, q, ng, C);
+hipError_t launch_coil_combine(const float2* G, const float2* maps, float2* q, long long ng, int C, long long nyz,
+                               long long xp, hipStream_t s) {
+    k_coil_combine<<<(unsigned)((ng + 255) / 256), 256, 0, s>>>(G, maps, q, ng, C, nyz, xp);
     return hipGetLastError();
 }
 

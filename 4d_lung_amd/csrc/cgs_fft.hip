@@ -463,7 +463,7 @@ struct FirstRegs {
             if ((NB % NT) == 0 || b < NB) {
                 const int col = b % NCOL, j = b / NCOL;
 #pragma unroll
-                for (int i = 0; i < R; ++i) v[p][i] = ld(j + i * (L / R), col);
+                for (int i = 0; i < R; ++i) v[p][i] = ld(i, j, col);   // line j + i * L / R
             }
         }
     }
@@ -512,10 +512,24 @@ __device__ __forceinline__ float2* elem(float2* base, Off off) {
     else return base + off;
 }
 
-template <int L, int W, int DIR, int CPB, class Off>
+// Addressing of the strided passes: line l = j + i * S of a column chunk
+// (i, S: radix index and stride of the first or last stage) at
+//   kOff32: one 32-bit byte offset from the chunk (column span below 4 GB);
+//   kOffSeg: the uniform base of line group i plus a 32-bit offset of
+//            j * ls + c (a group of S lines spans less than 4 GB, the whole
+//            column does not: x passes of the largest grids);
+//   kOff64: 64-bit element offsets.
+enum { kOff32 = 0, kOffSeg = 1, kOff64 = 2 };
+
+template <int AM>
+__device__ __forceinline__ float2* line_at(float2* g, long long ls, int i, int S, int j, int c) {
+    if constexpr (AM == kOff32) return elem<uint32_t>(g, (uint32_t)(j + i * S) * (uint32_t)ls + (uint32_t)c);
+    else if constexpr (AM == kOffSeg) return elem<uint32_t>(g + (long long)(i * S) * ls, (uint32_t)j * (uint32_t)ls + (uint32_t)c);
+    else return g + (long long)(j + i * S) * ls + c;
+}
+
+template <int L, int W, int DIR, int CPB, int AM>
 __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
-    // Off: type of the element offsets from the chunk's first column (32-bit
-    // where every byte offset of the line group fits in 32 bits)
     __shared__ float2 lds[L * W];
     __shared__ float2 twl[L];
     const long long nchunk = (a.ncols + W - 1) / W;
@@ -525,12 +539,12 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
     if (a.done && a.done[outer / a.outer_per_bin]) return;
     for (int e = threadIdx.x; e < L; e += CGS_THREADS) twl[e] = a.tw[e];
     float2* gout = a.g + outer * a.os;
-    const Off ls = (Off)a.ls;
+    const long long ls = a.ls;
     const long long ncols = a.ncols;
     constexpr int NS = plan_nstages(L);
     const long long c_first = grp * CPB;
     const long long c_end = min(c_first + CPB, nchunk);
-    if constexpr (NS == 1) {   // one radix stage: registers only
+    if constexpr (NS == 1) {   // one radix stage: registers only (line l = group l of one line)
         for (long long chunk = c_first; chunk < c_end; ++chunk) {
             float2* g = gout + chunk * W;
             const long long nh = ncols - chunk * W;
@@ -538,15 +552,17 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
                 lds, twl,
                 [&](int l, int c) {
                     float2 v = make_float2(0.0f, 0.0f);
-                    if (W <= nh || c < nh) v = *elem<Off>(g, (Off)l * ls + (Off)c);
+                    if (W <= nh || c < nh) v = *line_at<AM>(g, ls, l, 1, 0, c);
                     return alt_sign(v, l);
                 },
                 [&](int l, int c, float2 v, int, int) {
-                    if (W <= nh || c < nh) *elem<Off>(g, (Off)l * ls + (Off)c) = alt_sign(v, l);
+                    if (W <= nh || c < nh) *line_at<AM>(g, ls, l, 1, 0, c) = alt_sign(v, l);
                 });
         }
         return;
     }
+    constexpr int S0 = L / plan_radix(L, 0);        // first-stage line stride
+    constexpr int SL = plan_ls(L, NS - 1);          // last-stage line stride
     // columns past the end of the last chunk: loads clamped to its last
     // column (unconditional), values zeroed in registers, stores skipped
     int cmax = 0;
@@ -557,7 +573,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
     auto load_chunk = [&](FirstRegs<L, W, CGS_THREADS>& r, long long chunk) {
         float2* g = gout + chunk * W;
         const int cm = (int)min((long long)W, ncols - chunk * W) - 1;
-        r.load([&](int l, int c) { return *elem<Off>(g, (Off)l * ls + (Off)min(c, cm)); });
+        r.load([&](int i, int j, int c) { return *line_at<AM>(g, ls, i, S0, j, min(c, cm)); });
     };
     FirstRegs<L, W, CGS_THREADS> next;
     load_chunk(next, c_first);
@@ -568,8 +584,8 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
         if (chunk + 1 < c_end) load_chunk(next, chunk + 1);
         fft_mid_stages<L, 1, NS - 1, plan_radix(L, 0), DIR, W, W, CGS_THREADS>(lds, twl);
         float2* g = gout + chunk * W;
-        fft_last_stage<L, DIR, W, W, false, CGS_THREADS>(lds, twl, [&](int l, int c, float2 v, int, int) {
-            if (c <= cmax) *elem<Off>(g, (Off)l * ls + (Off)c) = alt_sign(v, l);
+        fft_last_stage<L, DIR, W, W, false, CGS_THREADS>(lds, twl, [&](int l, int c, float2 v, int, int i) {
+            if (c <= cmax) *line_at<AM>(g, ls, i, SL, l - i * SL, c) = alt_sign(v, l);
         });
         if (chunk + 1 < c_end) __syncthreads();
     }
@@ -618,7 +634,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     // block-uniform bases and 32-bit offsets within the line; coil indices
     // are clamped so that every load is unconditional (columns past the last
     // coil are zeroed when the tile is written to LDS)
-    float2* g = a.g + (bin * a.ng + cell0) * a.C;
+    float2* g = a.g + bin * a.bp + (xy / a.ny) * a.xp + (xy % a.ny) * (long long)(L * a.C);
     const int C = a.C;
     const float2* ml = HAS_P ? a.maps + cell0 * C : nullptr;
     // raw element: S (with p) or the coil image value (without)
@@ -705,7 +721,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
     }
     const int gx = (int)(xy / a.ny), gy = (int)(xy % a.ny);
     const bool xy_in = !a.crop || (gx >= a.lo[0] && gx <= a.hi[0] && gy >= a.lo[1] && gy <= a.hi[1]);
-    float2* g = a.g + (bin * a.ng + cell0) * a.C;
+    float2* g = a.g + bin * a.bp + gx * a.xp + gy * (long long)(L * a.C);
     const bool combine = a.maps != nullptr;
     const int C = a.C;
     const float2* ml = a.maps ? a.maps + cell0 * C : nullptr;
@@ -838,10 +854,16 @@ static inline bool plain_far(const PlainArgs& a) { return a.ls * (long long)size
 template <int L, int DIR, int CPB>
 static void plain_launch_cpb(const PlainArgs& a, dim3 grid, hipStream_t s) {
     constexpr int W = plain_w(L);
-    // largest byte offset from a chunk's first column
-    const long long span = ((long long)(L - 1) * a.ls + a.ncols) * (long long)sizeof(float2);
-    if (span < (1LL << 32)) k_fft_plain<L, W, DIR, CPB, uint32_t><<<grid, CGS_THREADS, 0, s>>>(a);
-    else k_fft_plain<L, W, DIR, CPB, long long><<<grid, CGS_THREADS, 0, s>>>(a);
+    constexpr int NS = plan_nstages(L);
+    // largest byte offset from a chunk's first column, and from the base of
+    // a line group of the first or last stage
+    const long long span = ((long long)(L - 1) * a.ls + W) * (long long)sizeof(float2);
+    constexpr int SG = NS == 1 ? 1 : (L / plan_radix(L, 0) > plan_ls(L, NS - 1) ? L / plan_radix(L, 0)
+                                                                               : plan_ls(L, NS - 1));
+    const long long seg = ((long long)(SG - 1) * a.ls + W) * (long long)sizeof(float2);
+    if (span < (1LL << 32)) k_fft_plain<L, W, DIR, CPB, kOff32><<<grid, CGS_THREADS, 0, s>>>(a);
+    else if (seg < (1LL << 32)) k_fft_plain<L, W, DIR, CPB, kOffSeg><<<grid, CGS_THREADS, 0, s>>>(a);
+    else k_fft_plain<L, W, DIR, CPB, kOff64><<<grid, CGS_THREADS, 0, s>>>(a);
 }
 
 template <int L, int DIR>

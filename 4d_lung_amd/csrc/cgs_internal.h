@@ -41,7 +41,15 @@ struct Geom {
     int nx, ny, nz;      // oversampled grid
     int C;               // coils
     long long ng;        // nx*ny*nz
+    long long xp;        // coil-grid x-plane pitch in elements (>= ny*nz*C)
+    long long bp;        // coil-grid pitch of one bin (nx*xp)
 };
+
+// Element (x, y, z, coil 0) of one bin's coil grid [x][y][z][c]: the x planes
+// are xp elements apart (padded, see coil_grid_pitch), y and z are dense.
+__host__ __device__ inline long long grid_at(const Geom& g, long long x, long long y, long long z) {
+    return x * g.xp + (y * g.nz + z) * g.C;
+}
 
 // One gridding-tile decomposition of the grid.
 struct Tiles {

@@ -20,7 +20,7 @@ struct PlainArgs {
 
 // z pass (lines along z, columns = coils) with fused prologue/epilogue.
 struct ZArgs {
-    float2* g;             // [bin][Ng][C]
+    float2* g;             // [bin][x][y][z][C], x planes xp apart, bins bp apart
     const float2* tw;
     const float2* p;       // [bin][Ng] image vector (zpre expand; zpost <p,q>)
     const float2* maps;    // [Ng][C]
@@ -30,6 +30,7 @@ struct ZArgs {
     float scale;           // zpost: 1e-3 / sqrt(Ng)
     int C;
     long long ng;
+    long long xp, bp;      // coil-grid x-plane and bin pitch (elements)
     long long nxy;         // Nx*Ny
     int ny;
     int crop;              // reduceFOV active
@@ -81,8 +82,8 @@ hipError_t launch_gather_data(const float2* data, long long ns, const int4* base
                               const float* dcf_full, const int* perm, float* dcf_sorted, float2* y, hipStream_t s);
 hipError_t launch_scatter_data(const float2* y, const int4* base, long long m, int C, long long ns,
                                float2* data, hipStream_t s);
-hipError_t launch_transpose_coils(const float2* in, float2* out, long long ng, int C, int to_interleaved,
-                                  hipStream_t s);
+hipError_t launch_transpose_coils(const float2* in, float2* out, long long ng, int C, long long nyz, long long xp,
+                                  int to_interleaved, hipStream_t s);
 
 // ---------------------------------------------------------- gridding
 // Samples of all bins are concatenated; bin b owns [bin_off[b], bin_off[b+1]).
@@ -162,7 +163,8 @@ hipError_t launch_cg_update_p(float2* p, const float2* r, long long n, int nbins
                               hipStream_t s);
 hipError_t launch_copy_rhs(float2* x, float2* r, float2* p, const float2* b, long long n, int nbins, hipStream_t s);
 hipError_t launch_fill_zero(float2* a, long long n, hipStream_t s);
-hipError_t launch_coil_combine(const float2* G, const float2* maps, float2* q, long long ng, int C, hipStream_t s);
+hipError_t launch_coil_combine(const float2* G, const float2* maps, float2* q, long long ng, int C, long long nyz,
+                               long long xp, hipStream_t s);
 hipError_t launch_scale_d(float2* a, long long n, double f, hipStream_t s);
 
 // ---------------------------------------------------------------- Voronoi

@@ -57,7 +57,8 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
         iz[o] = min(max(bs.z + o, 0), a.g.nz - 1);
     }
     const int C = a.g.C;
-    const float2* __restrict__ G = a.G + (long long)(bin - a.b0) * a.g.ng * C;
+    const float2* __restrict__ G = a.G + (long long)(bin - a.b0) * a.g.bp;
+    const long long ys = (long long)a.g.nz * C;
     const float d = a.s.dcf ? a.s.dcf[s] : 1.0f;
     const float thrf = a.thrf;
     for (int c = lane_c; c < C; c += CW) {
@@ -65,13 +66,13 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
         for (int ox = 0; ox < KW; ++ox) {
             const double wx = vx[ox];
             if (wx == 0.0) continue;
-            const long long xo = (long long)min(max(bs.x + ox, 0), a.g.nx - 1) * a.g.ny;
+            const long long xo = (long long)min(max(bs.x + ox, 0), a.g.nx - 1) * a.g.xp;
             float2 gv[KW][KW];
 #pragma unroll
             for (int oy = 0; oy < KW; ++oy) {
-                const long long row = (xo + iy[oy]) * a.g.nz;
+                const long long row = xo + iy[oy] * ys;
 #pragma unroll
-                for (int oz = 0; oz < KW; ++oz) gv[oy][oz] = G[(row + iz[oz]) * C + c];
+                for (int oz = 0; oz < KW; ++oz) gv[oy][oz] = G[row + iz[oz] * C + c];
             }
 #pragma unroll
             for (int oy = 0; oy < KW; ++oy) {
@@ -239,7 +240,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
         }
     }
     // --- write the tile (every cell of the grid belongs to exactly one tile)
-    float2* __restrict__ G = a.G + (long long)lbin * a.g.ng * C;
+    float2* __restrict__ G = a.G + (long long)lbin * a.g.bp;
 #pragma unroll
     for (int k = 0; k < NPT; ++k) {
         const int e = tid + k * CGS_THREADS;
@@ -247,13 +248,13 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
         if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && g * CG < C) {
             const int x = x0 + col / TY, y = y0 + col % TY;
             if (x < a.g.nx && y < a.g.ny) {
-                const long long row = ((long long)x * a.g.ny + y) * a.g.nz;
+                const long long row = grid_at(a.g, x, y, z0);
 #pragma unroll
                 for (int z = 0; z < TZ; ++z)
                     if (z0 + z < a.g.nz) {
 #pragma unroll
                         for (int q = 0; q < CG; ++q)
-                            if (g * CG + q < C) G[(row + z0 + z) * C + g * CG + q] = acc[k][q][z];
+                            if (g * CG + q < C) G[row + z * C + g * CG + q] = acc[k][q][z];
                     }
             }
         }
@@ -289,14 +290,14 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grad_k(GradKArgs a) {
                     const double vxy = v[ox] * v[KW + oy];
                     const double dxy = dv[ox] * v[KW + oy];
                     const double xdy = v[ox] * dv[KW + oy];
-                    const long long row = ((long long)xg * a.g.ny + yg) * a.g.nz;
+                    const long long row = grid_at(a.g, xg, yg, 0);
 #pragma unroll
                     for (int oz = 0; oz < KW; ++oz) {
                         const int zg = bs.z + oz;
                         if (zg < 0 || zg >= a.g.nz) continue;
                         const float w = (float)(vxy * v[2 * KW + oz]);
                         if (!(w >= a.thrf)) continue;
-                        const float2 av = a.A[(row + zg) * C + c];
+                        const float2 av = a.A[row + (long long)zg * C + c];
                         const float t = bv.x * av.x + bv.y * av.y;
                         g0 = fmaf(t, (float)(dxy * v[2 * KW + oz]), g0);
                         g1 = fmaf(t, (float)(xdy * v[2 * KW + oz]), g1);

@@ -280,15 +280,17 @@ __global__ void k_scatter_data(const float2* __restrict__ y, const int4* __restr
     data[(long long)c * ns + base[s].w] = y[e];
 }
 
-// [C][Ng] <-> [Ng][C] through a 64x(C) LDS tile.
+// [C][Ng] <-> [Ng][C]; the interleaved side's x planes are xp elements apart
+// (nyz = ny*nz cells per plane).
 __global__ void k_transpose_coils(const float2* __restrict__ in, float2* __restrict__ out, long long ng, int C,
-                                  int to_interleaved) {
+                                  long long nyz, long long xp, int to_interleaved) {
     const long long cell = (long long)blockIdx.x * 64 + (threadIdx.x % 64);
     const int c0 = threadIdx.x / 64;
+    const long long il = (cell / nyz) * xp + (cell % nyz) * C;
     for (int c = c0; c < C; c += blockDim.x / 64) {
         if (cell >= ng) continue;
-        if (to_interleaved) out[cell * C + c] = in[(long long)c * ng + cell];
-        else out[(long long)c * ng + cell] = in[cell * C + c];
+        if (to_interleaved) out[il + c] = in[(long long)c * ng + cell];
+        else out[(long long)c * ng + cell] = in[il + c];
     }
 }
 
@@ -362,9 +364,9 @@ hipError_t launch_scatter_data(const float2* y, const int4* base, long long m, i
     return hipGetLastError();
 }
 
-hipError_t launch_transpose_coils(const float2* in, float2* out, long long ng, int C, int to_interleaved,
-                                  hipStream_t s) {
-    k_transpose_coils<<<nblocks(ng, 64), 256, 0, s>>>(in, out, ng, C, to_interleaved);
+hipError_t launch_transpose_coils(const float2* in, float2* out, long long ng, int C, long long nyz, long long xp,
+                                  int to_interleaved, hipStream_t s) {
+    k_transpose_coils<<<nblocks(ng, 64), 256, 0, s>>>(in, out, ng, C, nyz, xp, to_interleaved);
     return hipGetLastError();
 }
 
