@@ -37,8 +37,9 @@ def log(msg):
 
 def algorithmic_bytes(name, ng, C, m, vs, touched=None):
     """HBM bytes one launch must move at minimum (complex64 = 8 B).  touched:
-    grid cells within the KB window of at least one sample (gridding reads or
-    writes only those; the whole grid when not given)."""
+    grid cells within the KB window of at least one sample, the only ones
+    degridding reads (the whole grid when not given); gridding writes every
+    cell, zeros included, since the inverse FFT reads the whole grid."""
     nt = ng if touched is None else touched
     per = {
         "fft_z_fwd": (16 * C + 12) * ng,             # p, S_c, kft in; coil grids out
@@ -46,7 +47,7 @@ def algorithmic_bytes(name, ng, C, m, vs, touched=None):
         "fft_x_inv": 16 * C * ng, "fft_y_inv": 16 * C * ng,
         "fft_z_inv": (16 * C + 20) * ng,             # grids, S_c, kft, p in; q out
         "degrid": 8 * C * nt + m * (16 + 8 * vs + 4 + 8 * C),
-        "grid": 8 * C * nt + m * (16 + 8 * vs + 8 * C),
+        "grid": 8 * C * ng + m * (16 + 8 * vs + 8 * C),
         "cg_update_xr": 48 * ng,
         "cg_update_p": 24 * ng,
         "cg_alpha": 0, "cg_beta": 0,
