@@ -74,6 +74,8 @@ def lib():
         "cgs_cg_sense_images": (I, [P, P, D, I, P, P, P]),
         "cgs_set_num_bins": (I, [P, I]),
         "cgs_set_bin_group": (I, [P, I]),
+        "cgs_set_pruning": (I, [P, I]),
+        "cgs_get_pruning": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_get_bin_group": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_set_bin_data": (I, [P, I, P, P]),
         "cgs_set_bin_samples": (I, [P, I, LL, P, P, P]),
@@ -262,6 +264,16 @@ class Context:
         g = ctypes.c_int()
         self._ck(self._L.cgs_get_bin_group(self.h, ctypes.byref(g)))
         return g.value
+
+    def set_pruning(self, mode):
+        """Column pruning of the bins' x passes: 0 off, 1 automatic, 2 on."""
+        self._ck(self._L.cgs_set_pruning(self.h, int(mode)))
+
+    def pruning(self):
+        """Whether the packed bins use column pruning."""
+        a = ctypes.c_int()
+        self._ck(self._L.cgs_get_pruning(self.h, ctypes.byref(a)))
+        return bool(a.value)
 
     def set_bin_data(self, b, data, dcf=None):
         d = _c64(data, self.ncoil * self.ns)

@@ -82,6 +82,15 @@ struct cgs_ctx {
     float2* p_yw = nullptr;
     int* p_tile_off = nullptr;
     bool p_alias = false;          // packed arrays alias bins[0]'s
+    // column pruning of the bins' x passes (built with the packed arrays)
+    int prune_mode = 1;            // 0 off, 1 when it skips enough, 2 always
+    bool prune_on = false;
+    long long pr_ngrp = 0, pr_gstride = 0;
+    uint8_t* d_gact = nullptr;     // [nbins][pr_ngrp]
+    int* d_glist = nullptr;        // [nbins][pr_gstride]
+    int* d_gcnt = nullptr;         // [nbins]
+    uint8_t* d_tneed = nullptr;    // [nbins][nty*ntz]
+    std::vector<int> h_gcnt;
 
     // workspace
     float2* d_grid = nullptr;
@@ -362,9 +371,84 @@ void free_packed(cgs_ctx* h) {
     dfree(h, h->p_dcf, mt);
     dfree(h, h->p_sbin, mt);
     dfree(h, h->p_yw, mt * h->C);
+    dfree(h, h->d_gact, h->nbins * h->pr_ngrp);
+    dfree(h, h->d_glist, h->nbins * h->pr_gstride);
+    dfree(h, h->d_gcnt, h->nbins);
+    dfree(h, h->d_tneed, (long long)h->nbins * h->tiles.nty * h->tiles.ntz);
+    h->prune_on = false;
+    h->pr_ngrp = h->pr_gstride = 0;
+    h->h_gcnt.clear();
     h->p_alias = false;
     h->packed = false;
     drop_graph(h);
+}
+
+// Column pruning of the bins' x passes.  A bin's samples reach the grid
+// cells (y, z) within the KB window of their nearest cell (clamped, as
+// degridding reads them) -- every x.  The x pass transforms column groups of
+// W*cpb consecutive (y, z, coil) columns; a group no window reaches is
+// skipped by the forward x pass (degridding never reads it) and written as
+// zeros by the inverse one (gridding leaves it zero), and gridding skips the
+// tiles whose (y, z) columns all lie in such groups.  Worth it for the sparse
+// line sets of respiratory bins; with mode 1 it is used only when it skips at
+// least a tenth of the groups.
+int build_prune(cgs_ctx* h) {
+    const long long mt = h->bin_off.back();
+    const int ny = h->N[1], nz = h->N[2], C = h->C;
+    const long long nyz = (long long)ny * nz;
+    int W = 0, cpb = 0;
+    plain_geometry(h->N[0], h->xp, &W, &cpb);
+    const long long gw = (long long)W * cpb;
+    const long long ngrp = (nyz * C + gw - 1) / gw;
+    const Tiles& t = h->tiles;
+    const int kw = h->kb.kw;
+    std::vector<int4> base(std::max(1LL, mt));
+    if (mt) CK(hipMemcpy(base.data(), h->p_base, sizeof(int4) * mt, hipMemcpyDeviceToHost));
+    std::vector<uint8_t> gact((size_t)h->nbins * ngrp, 0), tneed((size_t)h->nbins * t.nty * t.ntz, 0);
+    std::vector<std::vector<int>> lists(h->nbins);
+    std::vector<uint8_t> cell(nyz);
+    long long active = 0;
+    for (int b = 0; b < h->nbins; ++b) {
+        std::fill(cell.begin(), cell.end(), 0);
+        for (long long s = h->bin_off[b]; s < h->bin_off[b + 1]; ++s)
+            for (int oy = 0; oy < kw; ++oy) {
+                const long long y = std::min(std::max(base[s].y + oy, 0), ny - 1);
+                for (int oz = 0; oz < kw; ++oz) cell[y * nz + std::min(std::max(base[s].z + oz, 0), nz - 1)] = 1;
+            }
+        uint8_t* ga = gact.data() + (size_t)b * ngrp;
+        for (long long c = 0; c < nyz; ++c)
+            if (cell[c])
+                for (long long g = c * C / gw; g <= (c * C + C - 1) / gw; ++g) ga[g] = 1;
+        uint8_t* tn = tneed.data() + (size_t)b * t.nty * t.ntz;
+        for (long long g = 0; g < ngrp; ++g) {
+            if (!ga[g]) continue;
+            lists[b].push_back((int)g);
+            const long long c1 = std::min((g * gw + gw - 1) / C, nyz - 1);
+            for (long long c = g * gw / C; c <= c1; ++c) tn[(c / nz / t.ty) * t.ntz + (c % nz) / t.tz] = 1;
+        }
+        active += (long long)lists[b].size();
+    }
+    h->prune_on = h->prune_mode == 2 || (h->prune_mode == 1 && active * 10 <= 9 * ngrp * h->nbins);
+    if (!h->prune_on) return CGS_OK;
+    long long stride = 1;
+    for (auto& l : lists) stride = std::max<long long>(stride, (long long)l.size());
+    std::vector<int> glist((size_t)h->nbins * stride, 0);
+    h->h_gcnt.assign(h->nbins, 0);
+    for (int b = 0; b < h->nbins; ++b) {
+        std::copy(lists[b].begin(), lists[b].end(), glist.begin() + (size_t)b * stride);
+        h->h_gcnt[b] = (int)lists[b].size();
+    }
+    h->pr_ngrp = ngrp;
+    h->pr_gstride = stride;
+    CKS(dalloc(h, &h->d_gact, h->nbins * ngrp));
+    CKS(dalloc(h, &h->d_glist, h->nbins * stride));
+    CKS(dalloc(h, &h->d_gcnt, h->nbins));
+    CKS(dalloc(h, &h->d_tneed, (long long)h->nbins * t.nty * t.ntz));
+    CK(hipMemcpy(h->d_gact, gact.data(), gact.size(), hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->d_glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->d_gcnt, h->h_gcnt.data(), sizeof(int) * h->nbins, hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->d_tneed, tneed.data(), tneed.size(), hipMemcpyHostToDevice));
+    return CGS_OK;
 }
 
 // Binning + stable tile sort + KB tables of the active samples among traj[ns]
@@ -494,6 +578,7 @@ int pack_bins(cgs_ctx* h) {
     }
     CKS(dalloc(h, &h->p_yw, std::max(1LL, mt) * h->C));
     CK(hipStreamSynchronize(h->st));
+    CKS(build_prune(h));
     h->packed = true;
     h->cg_ready = false;
     return CGS_OK;
@@ -536,7 +621,8 @@ int ensure_vectors(cgs_ctx* h) {
 // -------------------------------------------------------------- operators
 // Forward z pass (+ y, x passes): coil grids of bins b0..b0+nb-1 hold the
 // centred FFT of (p * S_c or the grids themselves) * kft.
-int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const int* done) {
+// bins: the grids are those of packed bins b0.. (x-pass column pruning applies).
+int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const int* done, bool bins = true) {
     const int W = zpass_w(h->N[2], h->C);
     ZArgs z{};
     z.g = h->d_grid; z.tw = h->d_tw[2]; z.p = p; z.maps = p ? h->d_maps : nullptr;
@@ -556,21 +642,27 @@ int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const 
     PlainArgs x{};
     x.g = h->d_grid; x.os = h->bp; x.ls = h->xp; x.ncols = (long long)h->N[1] * h->N[2] * h->C;
     x.tw = h->d_tw[0]; x.outer_per_bin = 1; x.done = done;
+    if (bins && h->prune_on) {
+        x.glist = h->d_glist + (size_t)b0 * h->pr_gstride;
+        x.gcnt = h->d_gcnt + b0;
+        x.gstride = h->pr_gstride;
+    }
     {
         Prof t(h, "fft_x_fwd");
         CK(launch_fft_plain(h->N[0], -1, x, nb, h->st));
     }
-    (void)b0;
     return CGS_OK;
 }
 
 // Inverse x, y passes and the fused inverse z pass.  With q != null the
 // coils are combined with conj(S) into q (and <p, q> partials when p and part
 // are given); otherwise the grids receive the coil images.
-int run_inv_fft(cgs_ctx* h, int nb, float2* q, const float2* p, double* part, const int* done) {
+int run_inv_fft(cgs_ctx* h, int b0, int nb, float2* q, const float2* p, double* part, const int* done,
+                bool bins = true) {
     PlainArgs x{};
     x.g = h->d_grid; x.os = h->bp; x.ls = h->xp; x.ncols = (long long)h->N[1] * h->N[2] * h->C;
     x.tw = h->d_tw[0]; x.outer_per_bin = 1; x.done = done;
+    if (bins && h->prune_on) x.gact = h->d_gact + (size_t)b0 * h->pr_ngrp;
     {
         Prof t(h, "fft_x_inv");
         CK(launch_fft_plain(h->N[0], +1, x, nb, h->st));
@@ -621,6 +713,7 @@ GridArgs grid_args(cgs_ctx* h, int b0, int nb) {
     g.b0 = b0;
     g.nb = nb;
     g.done = nullptr;
+    g.tneed = h->prune_on ? h->d_tneed : nullptr;
     return g;
 }
 
@@ -641,7 +734,7 @@ int apply_normal_bins(cgs_ctx* h, int b0, int nb, bool use_done) {
         Prof t(h, "grid");
         CK(launch_grid(h->kb.kw, h->tcfg, g, h->st));
     }
-    CKS(run_inv_fft(h, nb, h->d_q + (size_t)b0 * h->ng, p, h->d_part, done));
+    CKS(run_inv_fft(h, b0, nb, h->d_q + (size_t)b0 * h->ng, p, h->d_part, done));
     return CGS_OK;
 }
 
@@ -935,7 +1028,7 @@ int cgs_nufft_bwd(cgs_ctx* h, const float* data, float* img) {
     GridArgs g = grid_args(h, 0, 1);
     g.Y = h->p_yw;
     CK(launch_grid(h->kb.kw, h->tcfg, g, h->st));
-    CKS(run_inv_fft(h, 1, nullptr, nullptr, nullptr, nullptr));
+    CKS(run_inv_fft(h, 0, 1, nullptr, nullptr, nullptr, nullptr));
     float2* out = nullptr;
     CKS(dalloc(h, &out, (size_t)h->C * h->ng));
     CK(launch_transpose_coils(h->d_grid, out, h->ng, h->C, (long long)h->N[1] * h->N[2], h->xp, 0, h->st));
@@ -971,6 +1064,20 @@ int cgs_set_num_bins(cgs_ctx* h, int nbins) {
     h->bins.resize(nbins);
     h->nbins = nbins;
     h->cg_ready = false;
+    return CGS_OK;
+}
+
+int cgs_set_pruning(cgs_ctx* h, int mode) {
+    if (!h || mode < 0 || mode > 2) return CGS_ERR_ARG;
+    if (mode == h->prune_mode) return CGS_OK;
+    h->prune_mode = mode;
+    free_packed(h);
+    return CGS_OK;
+}
+
+int cgs_get_pruning(const cgs_ctx* h, int* active) {
+    if (!h || !active) return CGS_ERR_ARG;
+    *active = h->prune_on ? 1 : 0;
     return CGS_OK;
 }
 
@@ -1075,7 +1182,7 @@ int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter) {
             Prof t(h, "grid");
             CK(launch_grid(h->kb.kw, h->tcfg, g, h->st));
         }
-        CKS(run_inv_fft(h, nb, h->d_r + o, nullptr, nullptr, nullptr));
+        CKS(run_inv_fft(h, b0, nb, h->d_r + o, nullptr, nullptr, nullptr));
         CK(launch_copy_rhs(h->d_x + o, h->d_r + o, h->d_p + o, h->d_r + o, h->ng, nb, h->st));
         CK(launch_dot_partials(h->d_r + o, h->d_r + o, h->ng, nb, h->nblk_upd, h->d_part2, h->st));
         CK(launch_cg_init(h->d_sc + b0, h->d_part2, h->nblk_upd, nb, thresh, h->d_done + b0, h->st));
@@ -1554,7 +1661,7 @@ int cgs_pts_set_coords(cgs_pts* p, long long m, const float* k_dev) {
 int cgs_pts_forward(cgs_pts* p, const float* x_dev, float* y_dev) {
     CKS(pts_check(p, true));
     cgs_ctx* h = p->h;
-    CKS(run_fwd_fft(h, 0, 1, (const float2*)x_dev, true, nullptr));
+    CKS(run_fwd_fft(h, 0, 1, (const float2*)x_dev, true, nullptr, false));
     GridArgs g = pts_grid_args(p);
     {
         Prof t(h, "degrid");
@@ -1573,21 +1680,21 @@ int cgs_pts_adjoint(cgs_pts* p, const float* y_dev, float* x_dev) {
         Prof t(h, "grid");
         CK(launch_grid(h->kb.kw, h->tcfg, g, h->st));
     }
-    CKS(run_inv_fft(h, 1, (float2*)x_dev, nullptr, nullptr, nullptr));
+    CKS(run_inv_fft(h, 0, 1, (float2*)x_dev, nullptr, nullptr, nullptr, false));
     return CGS_OK;
 }
 
 int cgs_pts_grad_coords_fwd(cgs_pts* p, const float* x_dev, const float* gy_dev, float* gk_dev) {
     CKS(pts_check(p, true));
     cgs_ctx* h = p->h;
-    CKS(run_fwd_fft(h, 0, 1, (const float2*)x_dev, true, nullptr));
+    CKS(run_fwd_fft(h, 0, 1, (const float2*)x_dev, true, nullptr, false));
     return pts_grad(p, (const float2*)gy_dev, 1.0f, gk_dev);
 }
 
 int cgs_pts_grad_coords_adj(cgs_pts* p, const float* y_dev, const float* gx_dev, float* gk_dev) {
     CKS(pts_check(p, true));
     cgs_ctx* h = p->h;
-    CKS(run_fwd_fft(h, 0, 1, (const float2*)gx_dev, true, nullptr));
+    CKS(run_fwd_fft(h, 0, 1, (const float2*)gx_dev, true, nullptr, false));
     const float s = (float)(1e-3 / std::sqrt((double)h->ng));
     return pts_grad(p, (const float2*)y_dev, s, gk_dev);
 }

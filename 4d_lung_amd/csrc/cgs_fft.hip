@@ -535,15 +535,35 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
     const long long nchunk = (a.ncols + W - 1) / W;
     const long long ngrp = (nchunk + CPB - 1) / CPB;
     const long long bid = blockIdx.x;
-    const long long grp = bid % ngrp, outer = bid / ngrp;
+    long long grp, outer;
+    if (a.glist) {   // only the listed column groups
+        outer = bid / a.gstride;
+        const long long k = bid % a.gstride;
+        if (k >= a.gcnt[outer]) return;
+        grp = a.glist[outer * a.gstride + k];
+    } else {
+        grp = bid % ngrp;
+        outer = bid / ngrp;
+    }
     if (a.done && a.done[outer / a.outer_per_bin]) return;
-    for (int e = threadIdx.x; e < L; e += CGS_THREADS) twl[e] = a.tw[e];
     float2* gout = a.g + outer * a.os;
     const long long ls = a.ls;
     const long long ncols = a.ncols;
     constexpr int NS = plan_nstages(L);
     const long long c_first = grp * CPB;
     const long long c_end = min(c_first + CPB, nchunk);
+    if (a.gact && !a.gact[outer * ngrp + grp]) {   // no sample reaches these columns: zeros
+        for (long long chunk = c_first; chunk < c_end; ++chunk) {
+            float2* g = gout + chunk * W;
+            const int cm = (int)min((long long)W, ncols - chunk * W) - 1;
+            for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
+                const int l = e / W, c = e % W;
+                if (c <= cm) g[(long long)l * ls + c] = make_float2(0.0f, 0.0f);
+            }
+        }
+        return;
+    }
+    for (int e = threadIdx.x; e < L; e += CGS_THREADS) twl[e] = a.tw[e];
     if constexpr (NS == 1) {   // one radix stage: registers only (line l = group l of one line)
         for (long long chunk = c_first; chunk < c_end; ++chunk) {
             float2* g = gout + chunk * W;
@@ -872,12 +892,17 @@ static void plain_launch(const PlainArgs& a, dim3 grid, hipStream_t s) {
     else plain_launch_cpb<L, DIR, plain_cpb(false)>(a, grid, s);
 }
 
+void plain_geometry(int L, long long ls, int* W, int* cpb) {
+    *W = plain_w(L);
+    *cpb = plain_cpb(ls * (long long)sizeof(float2) >= (1LL << 20));
+}
+
 hipError_t launch_fft_plain(int L, int dir, const PlainArgs& a, long long nouter, hipStream_t s) {
     const int W = plain_w(L);
     const long long nchunk = (a.ncols + W - 1) / W;
     const int cpb = plain_cpb(plain_far(a));
     const long long ngrp = (nchunk + cpb - 1) / cpb;
-    const dim3 grid((unsigned)(nouter * ngrp));
+    const dim3 grid((unsigned)(nouter * (a.glist ? a.gstride : ngrp)));
     switch (L) {
 #define CGS_CASE(n)                                            \
     case n:                                                    \

@@ -16,7 +16,16 @@ struct PlainArgs {
     const float2* tw;      // exp(-2 pi i n / L), n < L
     long long outer_per_bin;
     const int* done;       // per-bin "solve finished" flags of the launch's bins, or null
+    // Column groups (the columns one block transforms) of a bin's x pass that
+    // its samples never reach can be skipped (outer = bin, x passes only):
+    const int* glist;      // forward: [bin][gstride] numbers of the groups to transform
+    const int* gcnt;       // forward: [bin] their count; the launch has nouter * gstride blocks
+    long long gstride;
+    const uint8_t* gact;   // inverse: [bin][groups] 1 = transform, 0 = write zeros (no reads)
 };
+
+// Column group of the strided passes: W columns per chunk, cpb chunks per block.
+void plain_geometry(int L, long long ls, int* W, int* cpb);
 
 // z pass (lines along z, columns = coils) with fused prologue/epilogue.
 struct ZArgs {
@@ -109,6 +118,8 @@ struct GridArgs {
     float thrf;            // KbParams::thr_up
     int b0, nb;
     const int* done;       // [nbins_total] flags, or null
+    const uint8_t* tneed;  // gridding: [nbins_total][nty*ntz] tile columns to write (others are
+                           // never read: their x-pass groups are zero-filled), or null = all
 };
 
 hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s);

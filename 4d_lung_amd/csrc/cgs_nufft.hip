@@ -117,6 +117,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     const int itz = tile % a.t.ntz;
     const int ity = (tile / a.t.ntz) % a.t.nty;
     const int itx = tile / (a.t.ntz * a.t.nty);
+    if (a.tneed && !a.tneed[(long long)bin * (a.t.nty * a.t.ntz) + ity * a.t.ntz + itz]) return;
     const int x0 = itx * TX, y0 = ity * TY, z0 = itz * TZ;
     const int C = a.g.C;
     const int* __restrict__ off = a.tile_off + (long long)bin * (ntiles + 1);

@@ -107,6 +107,15 @@ int cgs_set_bin_data(cgs_ctx* h, int bin, const float* data, const float* dcf);
 int cgs_set_bin_group(cgs_ctx* h, int max_bins);
 /* Bins per batched launch chosen by the last cgs_cg_prepare. */
 int cgs_get_bin_group(const cgs_ctx* h, int* bins_per_launch);
+/* Column pruning of the bins' x passes: column groups of the grid that no
+ * sample's KB window reaches are skipped by the forward x pass and written as
+ * zeros by the inverse one, and gridding skips the tiles of those columns
+ * (same images; pays for the sparse line sets of respiratory bins).  mode 0:
+ * off; 1 (default): on when it skips at least a tenth of the groups; 2: on.
+ * Takes effect when the bins are next packed.  No reference counterpart. */
+int cgs_set_pruning(cgs_ctx* h, int mode);
+/* 1 if the packed bins use column pruning. */
+int cgs_get_pruning(const cgs_ctx* h, int* active);
 /* Compact form: m acquired samples with original indices idx[m] (any order),
  * values y[c][m] complex and optional weights dcf[m]. */
 int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, const float* y, const float* dcf);
