@@ -76,6 +76,7 @@ def lib():
         "cgs_set_bin_group": (I, [P, I]),
         "cgs_set_pruning": (I, [P, I]),
         "cgs_get_pruning": (I, [P, ctypes.POINTER(ctypes.c_int)]),
+        "cgs_get_prune_fractions": (I, [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "cgs_get_bin_group": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_set_bin_data": (I, [P, I, P, P]),
         "cgs_set_bin_samples": (I, [P, I, LL, P, P, P]),
@@ -274,6 +275,13 @@ class Context:
         a = ctypes.c_int()
         self._ck(self._L.cgs_get_pruning(self.h, ctypes.byref(a)))
         return bool(a.value)
+
+    def prune_fractions(self, b=0):
+        """(x-pass column groups transformed, gridding tile columns written)
+        as fractions, for bin b of the packed bins."""
+        g, t = ctypes.c_double(), ctypes.c_double()
+        self._ck(self._L.cgs_get_prune_fractions(self.h, int(b), ctypes.byref(g), ctypes.byref(t)))
+        return g.value, t.value
 
     def set_bin_data(self, b, data, dcf=None):
         d = _c64(data, self.ncoil * self.ns)

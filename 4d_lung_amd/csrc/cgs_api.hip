@@ -91,6 +91,7 @@ struct cgs_ctx {
     int* d_gcnt = nullptr;         // [nbins]
     uint8_t* d_tneed = nullptr;    // [nbins][nty*ntz]
     std::vector<int> h_gcnt;
+    std::vector<double> pr_gfrac, pr_tfrac;   // per bin: groups transformed, tile columns written
 
     // workspace
     float2* d_grid = nullptr;
@@ -378,6 +379,8 @@ void free_packed(cgs_ctx* h) {
     h->prune_on = false;
     h->pr_ngrp = h->pr_gstride = 0;
     h->h_gcnt.clear();
+    h->pr_gfrac.clear();
+    h->pr_tfrac.clear();
     h->p_alias = false;
     h->packed = false;
     drop_graph(h);
@@ -429,7 +432,14 @@ int build_prune(cgs_ctx* h) {
         active += (long long)lists[b].size();
     }
     h->prune_on = h->prune_mode == 2 || (h->prune_mode == 1 && active * 10 <= 9 * ngrp * h->nbins);
+    h->pr_gfrac.assign(h->nbins, 1.0);
+    h->pr_tfrac.assign(h->nbins, 1.0);
     if (!h->prune_on) return CGS_OK;
+    for (int b = 0; b < h->nbins; ++b) {
+        const uint8_t* tn = tneed.data() + (size_t)b * t.nty * t.ntz;
+        h->pr_gfrac[b] = (double)lists[b].size() / (double)ngrp;
+        h->pr_tfrac[b] = (double)std::count(tn, tn + t.nty * t.ntz, (uint8_t)1) / (double)(t.nty * t.ntz);
+    }
     long long stride = 1;
     for (auto& l : lists) stride = std::max<long long>(stride, (long long)l.size());
     std::vector<int> glist((size_t)h->nbins * stride, 0);
@@ -1078,6 +1088,14 @@ int cgs_set_pruning(cgs_ctx* h, int mode) {
 int cgs_get_pruning(const cgs_ctx* h, int* active) {
     if (!h || !active) return CGS_ERR_ARG;
     *active = h->prune_on ? 1 : 0;
+    return CGS_OK;
+}
+
+int cgs_get_prune_fractions(const cgs_ctx* h, int bin, double* groups, double* tiles) {
+    if (!h || bin < 0 || bin >= h->nbins || !groups || !tiles) return CGS_ERR_ARG;
+    if (!h->packed || (int)h->pr_gfrac.size() != h->nbins) return CGS_ERR_STATE;
+    *groups = h->pr_gfrac[bin];
+    *tiles = h->pr_tfrac[bin];
     return CGS_OK;
 }
 

@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 runs of bench.py (kernel trace and HBM counters).
 
-    python 4d_lung_amd/profiling.py TRACE_DIR FETCH_DIR WRITE_DIR WORKLOAD OUT_DIR
+    python 4d_lung_amd/profiling.py TRACE_DIR FETCH_DIR WRITE_DIR WORKLOAD OUT_DIR [BINS_PER_LAUNCH]
 
 TRACE_DIR holds ``run_kernel_trace.csv`` of ``rocprofv3 --kernel-trace
 --stats -f csv -o run``; FETCH_DIR / WRITE_DIR hold ``run_counter_collection.csv``
@@ -12,7 +12,9 @@ written by the runtime's copy kernel).  Per stage the median over the run's
 launches is reported (setup launches of the same kernels are outnumbered by
 the CG iterations).  Writes
 ``OUT_DIR/kernels_WORKLOAD.md`` (durations) and ``OUT_DIR/traffic_WORKLOAD.txt``
-(bytes per launch, read back by bench.py for ``roofline.traffic``).
+(bytes per launch, read back by bench.py for ``roofline.traffic``).  With
+BINS_PER_LAUNCH (binned workloads profiled with every launch covering that
+many bins) the traffic is written per bin.
 """
 import collections
 import csv
@@ -84,6 +86,7 @@ def counter_bytes(pmc_dir, counter):
 
 def main(argv):
     trace_dir, fetch_dir, write_dir, workload, out_dir = argv[1:6]
+    nb = int(argv[6]) if len(argv) > 6 else 1
     dur, names = kernel_durations(trace_dir)
     fetch, fcal, fs = counter_bytes(fetch_dir, "FETCH_SIZE")
     write, wcal, ws = counter_bytes(write_dir, "WRITE_SIZE")
@@ -91,15 +94,19 @@ def main(argv):
     order = sorted(dur, key=lambda k: -med(dur[k]))
     os.makedirs(out_dir, exist_ok=True)
     with open(os.path.join(out_dir, "traffic_%s.txt" % workload), "w") as f:
-        f.write("# HBM bytes per launch of the CG-iteration kernels, workload %s\n" % workload)
+        if nb > 1:
+            f.write("# HBM bytes per bin of the CG-iteration kernels (launches of %d bins), workload %s\n"
+                    % (nb, workload))
+        else:
+            f.write("# HBM bytes per launch of the CG-iteration kernels, workload %s\n" % workload)
         f.write("# FETCH_SIZE / WRITE_SIZE (KiB per dispatch), median over the run's launches,\n")
         f.write("# scaled by the 1 GiB calibration copy: it read %.0f KiB (x%.4f) and wrote %.0f KiB (x%.4f)\n"
                 % (fcal, fs, wcal, ws))
         f.write("# stage fetch_bytes write_bytes\n")
         for st in order:
             if st in fetch and st in write:
-                fb = med(fetch[st])
-                wb = med(write[st])
+                fb = med(fetch[st]) / nb
+                wb = med(write[st]) / nb
                 f.write("%s %.6g %.6g\n" % (st, fb, wb))
     with open(os.path.join(out_dir, "kernels_%s.md" % workload), "w") as f:
         f.write("Per CG stage (kernel trace):\n\n")
@@ -118,17 +125,21 @@ def main(argv):
 
 
 def read_traffic(path):
-    """{stage: (fetch_bytes, write_bytes)} from a traffic_*.txt summary."""
+    """({stage: (fetch_bytes, write_bytes)}, per_bin) from a traffic_*.txt
+    summary; per_bin when the figures are per bin of a binned workload."""
     out = {}
+    per_bin = False
     if not os.path.exists(path):
-        return out
+        return out, per_bin
     with open(path) as f:
         for line in f:
+            if line.startswith("# HBM bytes per bin"):
+                per_bin = True
             if line.startswith("#") or not line.strip():
                 continue
             st, fb, wb = line.split()
             out[st] = (float(fb), float(wb))
-    return out
+    return out, per_bin
 
 
 if __name__ == "__main__":

@@ -35,19 +35,23 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(name, ng, C, m, vs, touched=None):
+def algorithmic_bytes(name, ng, C, m, vs, touched=None, groups=1.0, tiles=1.0):
     """HBM bytes one launch must move at minimum (complex64 = 8 B).  touched:
     grid cells within the KB window of at least one sample, the only ones
-    degridding reads (the whole grid when not given); gridding writes every
-    cell, zeros included, since the inverse FFT reads the whole grid."""
+    degridding reads (the whole grid when not given).  groups / tiles: the
+    fractions of the x pass's column groups that column pruning transforms
+    and of the tile columns gridding writes (the forward x pass touches only
+    those groups, the inverse one reads them and writes every column;
+    without pruning gridding writes every cell, zeros included, since the
+    inverse FFT reads the whole grid)."""
     nt = ng if touched is None else touched
     per = {
         "fft_z_fwd": (16 * C + 12) * ng,             # p, S_c, kft in; coil grids out
-        "fft_y_fwd": 16 * C * ng, "fft_x_fwd": 16 * C * ng,
-        "fft_x_inv": 16 * C * ng, "fft_y_inv": 16 * C * ng,
+        "fft_y_fwd": 16 * C * ng, "fft_x_fwd": 16 * C * ng * groups,
+        "fft_x_inv": 8 * C * ng * (1.0 + groups), "fft_y_inv": 16 * C * ng,
         "fft_z_inv": (16 * C + 20) * ng,             # grids, S_c, kft, p in; q out
         "degrid": 8 * C * nt + m * (16 + 8 * vs + 4 + 8 * C),
-        "grid": 8 * C * ng + m * (16 + 8 * vs + 8 * C),
+        "grid": 8 * C * ng * tiles + m * (16 + 8 * vs + 8 * C),
         "cg_update_xr": 48 * ng,
         "cg_update_p": 24 * ng,
         "cg_alpha": 0, "cg_beta": 0,
@@ -78,22 +82,28 @@ def stage_rates(rep, n_iter, bytes_per_iter):
     out = {}
     for name, (cnt, ms) in rep.items():
         b = bytes_per_iter.get(name, 0) * n_iter
-        out[name] = dict(count=cnt, avg_ms=ms / cnt, bytes=b / cnt if b else 0,
+        out[name] = dict(count=cnt, avg_ms=ms / cnt, bytes=b / cnt if b else 0, n_iter=n_iter,
                          gbs=(b / (ms * 1e-3) / 1e9) if b else None)
     return out
 
 
-def roofline_of(kernels, workload):
-    """Roofline object of the dominant stage (largest total time)."""
+def roofline_of(kernels, workload, nbins=1):
+    """Roofline object of the dominant stage (largest total time).  The HBM
+    traffic comes from the committed rocprofv3 summary of the workload
+    (per bin for binned workloads, scaled to the bins of one launch here)."""
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count"])
     prof_mod = importlib.import_module("4d_lung_amd.profiling")
     traffic_src = os.path.join("profiles", "traffic_%s.txt" % workload)
-    traffic = prof_mod.read_traffic(os.path.join(ROOT, traffic_src))
+    traffic, per_bin = prof_mod.read_traffic(os.path.join(ROOT, traffic_src))
     dk = kernels[dom]
+    t = None
+    if dom in traffic:
+        t = traffic[dom][0] + traffic[dom][1]
+        if per_bin:
+            t *= nbins * kernels[dom]["n_iter"] / kernels[dom]["count"]
     return {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": (dk["gbs"] / HBM_PEAK_GBS) if dk["gbs"] else None,
-            "traffic": traffic[dom][0] + traffic[dom][1] if dom in traffic else None,
-            "traffic_source": traffic_src if dom in traffic else None,
+            "traffic": t, "traffic_source": traffic_src if t is not None else None,
             "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"]}
 
 
@@ -182,15 +192,16 @@ def bench_binned(args, cfg, pkg, torch):
     value = B * K / secs
     log("batched: %d iterations of %d bins (%d per launch) in %.3f s: %.2f bin-iterations/s"
         % (K, B, ctx.bin_group(), secs, value))
-    ctx.cg_prepare(0.0, K - 1)
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    ctx.cg_iterate(pkg.SOLVE_PLAIN, K)
-    torch.cuda.synchronize()
-    plain_secs = time.perf_counter() - tp
-    plain_value = B * K / plain_secs
-    log("plain (bin after bin, host residual read per iteration): %.2f bin-iterations/s, batched/plain %.2fx"
-        % (plain_value, value / plain_value))
+    plain_value = None
+    if not args.no_plain:
+        ctx.cg_prepare(0.0, K - 1)
+        torch.cuda.synchronize()
+        tp = time.perf_counter()
+        ctx.cg_iterate(pkg.SOLVE_PLAIN, K)
+        torch.cuda.synchronize()
+        plain_value = B * K / (time.perf_counter() - tp)
+        log("plain (bin after bin, host residual read per iteration): %.2f bin-iterations/s, batched/plain %.2fx"
+            % (plain_value, value / plain_value))
     # per-stage HIP-event timing over 2 batched iterations
     n_prof = 2
     ctx.cg_prepare(0.0, K - 1)
@@ -202,8 +213,9 @@ def bench_binned(args, cfg, pkg, torch):
     kw = ctx.kernel_info()[0]
     vs = (3 * kw + 1) & ~1
     touched = [touched_cells(traj[idx], grid, kw) for idx, _, _ in bins]
-    per_iter = {name: sum(algorithmic_bytes(name, ng, C, m, vs, t) for m, t in zip(counts, touched))
-                for name in rep}
+    pruned = [ctx.prune_fractions(b) for b in range(B)]
+    per_iter = {name: sum(algorithmic_bytes(name, ng, C, m, vs, t, gf, tf)
+                          for m, t, (gf, tf) in zip(counts, touched, pruned)) for name in rep}
     kernels = stage_rates(rep, n_prof, per_iter)
     for name in sorted(kernels, key=lambda k: -kernels[k]["avg_ms"] * kernels[k]["count"]):
         kk = kernels[name]
@@ -226,10 +238,13 @@ def bench_binned(args, cfg, pkg, torch):
         "config": {"workload": args.workload, "image": list(image), "grid": list(grid), "coils": C, "bins": B,
                    "samples_per_bin": counts, "acceleration": list(cfg["R"]), "cg_iterations": K,
                    "gridding_tile": list(ctx.tile_config()[1]), "tile_config": int(ctx.tile_config()[0]),
-                   "bins_per_launch": ctx.bin_group(), "density": "voronoi per bin" if voronoi else "none"},
+                   "bins_per_launch": ctx.bin_group(), "density": "voronoi per bin" if voronoi else "none",
+                   "column_pruning": ctx.pruning(),
+                   "x_groups_transformed": round(float(np.mean([p[0] for p in pruned])), 4),
+                   "grid_tiles_written": round(float(np.mean([p[1] for p in pruned])), 4)},
         "plain_path_value": plain_value,
-        "batched_over_plain": value / plain_value,
-        "roofline": roofline_of(kernels, args.workload),
+        "batched_over_plain": value / plain_value if plain_value else None,
+        "roofline": roofline_of(kernels, args.workload, B),
         "algorithmic_gbs_per_iteration": total_bytes * value / B / 1e9,
         "voronoi_seconds": voronoi_s,
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_iteration": v["count"] // n_prof,
@@ -321,6 +336,7 @@ def main():
                     help="binned workloads: iterations of the oracle check (default: the workload's own, else "
                          "--steps); with fewer than --steps the checked bins are solved again for that many")
     ap.add_argument("--bin-group", type=int, default=-1, help="binned workloads: bins per launch (-1: autotune)")
+    ap.add_argument("--no-plain", action="store_true", help="binned workloads: skip timing the plain path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (the reference kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
@@ -439,16 +455,17 @@ def main():
     ctx.profile(False)
     kernels = {}
     touched = touched_cells(traj[np.flatnonzero(data[0] != 0)], grid, kw)
+    gfrac, tfrac = ctx.prune_fractions(0)
     for name, (cnt, ms) in rep.items():
         avg = ms / cnt
-        b = algorithmic_bytes(name, ng, C, m, vs, touched)
+        b = algorithmic_bytes(name, ng, C, m, vs, touched, gfrac, tfrac)
         kernels[name] = dict(count=cnt, avg_ms=avg, gbs=(b / (avg * 1e-3) / 1e9) if b else None, bytes=b)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["count"])
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
     # this workload (profiles/traffic_<workload>.txt, scaled by a copy kernel)
     prof_mod = importlib.import_module("4d_lung_amd.profiling")
     traffic_src = os.path.join("profiles", "traffic_%s.txt" % args.workload)
-    traffic = prof_mod.read_traffic(os.path.join(ROOT, traffic_src))
+    traffic, _ = prof_mod.read_traffic(os.path.join(ROOT, traffic_src))
     dk = kernels[dom]
     for name in sorted(kernels, key=lambda k: -kernels[k]["avg_ms"]):
         kk = kernels[name]
@@ -469,6 +486,7 @@ def main():
         "data": "synthetic (seeded phantom x smooth coil maps + noise, wave-CAIPI)",
         "config": {"workload": args.workload, "image": list(image), "grid": list(grid), "coils": C,
                    "samples_per_coil": int(m), "acceleration": list(cfg["R"]), "cg_iterations": K,
+                   "column_pruning": ctx.pruning(),
                    "gridding_tile": list(ctx.tile_config()[1]), "tile_config": int(ctx.tile_config()[0])},
         "plain_path_value": plain_value,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": dk["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -116,6 +116,10 @@ int cgs_get_bin_group(const cgs_ctx* h, int* bins_per_launch);
 int cgs_set_pruning(cgs_ctx* h, int mode);
 /* 1 if the packed bins use column pruning. */
 int cgs_get_pruning(const cgs_ctx* h, int* active);
+/* Fractions of bin's x-pass column groups that are transformed and of its
+ * gridding tile columns that are written (1 and 1 without pruning); valid
+ * while the bins stay packed (after cgs_cg_prepare). */
+int cgs_get_prune_fractions(const cgs_ctx* h, int bin, double* groups, double* tiles);
 /* Compact form: m acquired samples with original indices idx[m] (any order),
  * values y[c][m] complex and optional weights dcf[m]. */
 int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, const float* y, const float* dcf);
