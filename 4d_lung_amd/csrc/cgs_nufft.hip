@@ -136,6 +136,18 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
 
     const int tid = threadIdx.x;
     const int lane = tid % 64, wid = tid / 64;
+    // grid rows x (of this tile) that this wave's threads own, conservatively
+    int wxlo = TX, wxhi = -1;
+#pragma unroll
+    for (int k = 0; k < NPT; ++k) {
+        const int e0 = wid * 64 + k * CGS_THREADS;
+        if (e0 < NPAIR) {
+            const int e1 = min(e0 + 63, NPAIR - 1);
+            wxlo = min(wxlo, (e0 / NG) / TY);
+            wxhi = max(wxhi, (e1 / NG) / TY);
+        }
+    }
+    const int gxlo = x0 + wxlo, gxhi = x0 + wxhi;
     const int tz_lo = itz > 0 ? itz - 1 : 0;
     const int tz_hi = itz + 1 < a.t.ntz ? itz + 1 : a.t.ntz - 1;
 
@@ -204,8 +216,13 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
             }
             if (tid < nsub) sbase[tid] = a.s.base[list[j0 + tid]];
             __syncthreads();
-            // --- accumulate
-            for (int j = 0; j < nsub; ++j) {
+            // --- accumulate, walking only the candidates whose window covers
+            // one of the wave's rows (the others add nothing to its cells),
+            // in ascending order
+            unsigned long long todo = __ballot(lane < nsub && sbase[lane].x <= gxhi && sbase[lane].x + KW > gxlo);
+            while (todo) {
+                const int j = __builtin_ctzll(todo);
+                todo &= todo - 1ULL;
                 const int4 bs = sbase[j];
                 const double* v = svals + j * VS;
 #pragma unroll
