@@ -1463,10 +1463,16 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     if (cnt[0] > 0) {
-        // second pass: at most 2048 threads, each with a large cell workspace
+        // second pass: one thread per listed cell, each with a large cell
+        // workspace (about 330 KB) in device memory, as many as a quarter of
+        // the free memory (at most 16 GB) holds
         size_t cell_b = 0, scr_b = 0;
         voronoi_large_sizes(&cell_b, &scr_b);
-        const int nthreads = (int)std::min<long long>(((long long)cnt[0] + 63) / 64 * 64, 2048);
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
+        const long long budget = std::min<long long>((long long)(fr / 4), 16LL << 30);
+        const long long fit = std::max<long long>(2048, budget / (long long)(cell_b + scr_b) / 64 * 64);
+        const int nthreads = (int)std::min<long long>(((long long)cnt[0] + 63) / 64 * 64, fit);
         char *cells = nullptr, *scr = nullptr;
         CKS(dalloc(h, &cells, cell_b * nthreads));
         CKS(dalloc(h, &scr, scr_b * nthreads));

@@ -13,7 +13,8 @@
 // Voronoi region reaches more than one hash cell beyond the point cloud.  The
 // volume is the reference's face sum area * |q - p| / 6 with areas as
 // triangle fans from the first vertex.  A cell is a pool of vertices and
-// faces as rings of pool indices (about 8.5 KB per thread).
+// faces as rings of pool indices (about 12.7 KB per thread with the
+// first-pass limits).
 //
 // Built with -ffp-contract=off; every operation is a separately rounded
 // IEEE double operation in a fixed order, so the weights are reproducible
@@ -24,9 +25,6 @@
 namespace cgs {
 namespace vor {
 
-constexpr int FMAX = 48;   // faces of a cell
-constexpr int VMAX = 32;   // vertices of one face
-constexpr int PMAX = 96;   // distinct vertices of a cell (a simple polyhedron with F faces has <= 2F - 4)
 constexpr int RMAX = 10;   // hash shells visited at most
 
 // Fixed capacities of a cell: faces, vertices of one face, distinct vertices
@@ -41,7 +39,7 @@ struct Limits {
     static constexpr int PMAX = P;
     using Idx = I;
 };
-using SmallLimits = Limits<48, 32, 96, uint8_t>;
+using SmallLimits = Limits<64, 48, 128, uint8_t>;
 using LargeLimits = Limits<384, 256, 2048, uint16_t>;
 
 struct V3 {
@@ -373,9 +371,9 @@ __global__ __launch_bounds__(64) void k_voronoi_large(const double* __restrict__
 
 hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* redo, int* nredo,
                           hipStream_t s) {
-    // Each thread holds a small-limit cell of about 8.5 KB in private
+    // Each thread holds a small-limit cell of about 12.7 KB in private
     // (scratch) memory, which the runtime backs for every wave of the launch.
-    // The grid is capped at two blocks per CU (512 waves, about 300 MB of
+    // The grid is capped at two blocks per CU (512 waves, about 420 MB of
     // scratch) and 64 KB of dynamic LDS per block keeps at most two of them
     // resident on a CU.
     const long long blocks = std::min<long long>((np + 63) / 64, 512);

@@ -46,7 +46,7 @@ struct Limits {
     static constexpr int PMAX = P;
     using Idx = I;
 };
-using SmallLimits = Limits<48, 32, 96, uint8_t>;
+using SmallLimits = Limits<64, 48, 128, uint8_t>;
 using LargeLimits = Limits<384, 256, 2048, uint16_t>;
 
 struct V3 {
