@@ -1,0 +1,83 @@
+"""Host-side pieces of the measurement and tuning code (no GPU): the bin
+grouping candidates of the autotuner, the grid cells the gridding kernels
+touch (against a brute-force count), the algorithmic byte counts (against the
+per-iteration formula they add up to) and the rocprofv3 traffic summaries."""
+import importlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import bench  # noqa: E402
+
+tune = importlib.import_module("4d_lung_amd.autotune")
+prof = importlib.import_module("4d_lung_amd.profiling")
+
+
+def test_group_candidates_are_equal_splits():
+    assert tune.group_candidates(20, 10) == [10, 5, 3, 2, 1]
+    assert tune.group_candidates(8, 8) == [8, 4, 2, 1]
+    assert tune.group_candidates(20, 7) == [7, 4, 2, 1]
+    assert tune.group_candidates(1, 1) == [1]
+    for n in range(1, 30):
+        for g in range(1, n + 1):
+            for e in tune.group_candidates(n, g):
+                ngroups = -(-n // e)
+                assert e <= g and -(-n // ngroups) == e   # an equal split of n bins
+
+
+def test_touched_cells_brute_force():
+    rng = np.random.default_rng(1)
+    for grid in [(12, 10, 9), (8, 16, 6)]:
+        k = rng.uniform(0, 1, (40, 3))
+        k[:3] = [[0.0, 0.0, 0.0], [1.0, 1.0, 1.0], [0.0, 1.0, 0.5]]   # windows clamped at the edges
+        cells = set()
+        for p in k:
+            n = [int(np.floor((grid[a] - 1) * p[a] + 0.5)) for a in range(3)]
+            for o in np.ndindex(5, 5, 5):
+                cells.add(tuple(min(max(n[a] + o[a] - 2, 0), grid[a] - 1) for a in range(3)))
+        assert bench.touched_cells(k, grid, 5) == len(cells)
+
+
+def test_algorithmic_bytes_add_up_to_the_iteration_formula():
+    """Per iteration and bin without pruning: C(56 Ng + ...) of the design's
+    byte model, here (112C + 104) Ng + M (16C + 292) with the FP64 tables."""
+    ng, C, m, vs = 1000, 16, 300, 16
+    stages = ["fft_z_fwd", "fft_y_fwd", "fft_x_fwd", "degrid", "grid", "fft_x_inv", "fft_y_inv", "fft_z_inv",
+              "cg_update_xr", "cg_update_p", "cg_alpha", "cg_beta"]
+    tot = sum(bench.algorithmic_bytes(s, ng, C, m, vs) for s in stages)
+    assert tot == (112 * C + 104) * ng + m * (16 * C + 292)
+    # pruning removes the skipped groups' reads and writes from the forward x
+    # pass, their reads from the inverse one, and the skipped tiles' writes
+    full = {s: bench.algorithmic_bytes(s, ng, C, m, vs) for s in stages}
+    part = {s: bench.algorithmic_bytes(s, ng, C, m, vs, None, 0.25, 0.5) for s in stages}
+    assert part["fft_x_fwd"] == 0.25 * full["fft_x_fwd"]
+    assert part["fft_x_inv"] == 0.625 * full["fft_x_inv"]
+    assert full["grid"] - part["grid"] == 0.5 * 8 * C * ng
+    # degridding reads only the touched cells
+    assert bench.algorithmic_bytes("degrid", ng, C, m, vs, 100) == 8 * C * 100 + m * (16 + 8 * vs + 4 + 8 * C)
+
+
+def test_traffic_summary_round_trip(tmp_path):
+    p = tmp_path / "traffic_x.txt"
+    p.write_text("# HBM bytes per bin of the CG-iteration kernels (launches of 5 bins), workload x\n"
+                 "# stage fetch_bytes write_bytes\nfft_y_fwd 2e9 2e9\ngrid 1e8 7e9\n")
+    t, per_bin = prof.read_traffic(str(p))
+    assert per_bin and t == {"fft_y_fwd": (2e9, 2e9), "grid": (1e8, 7e9)}
+    t, per_bin = prof.read_traffic(str(tmp_path / "missing.txt"))
+    assert t == {} and not per_bin
+    t, per_bin = prof.read_traffic(os.path.join(ROOT, "profiles", "traffic_single_160x160x112.txt"))
+    assert not per_bin and set(t) >= {"grid", "degrid", "fft_x_fwd", "fft_z_inv"}
+
+
+def test_stage_names_of_kernel_trace():
+    cnt = {-1: 0, 1: 0}
+    names = ["void cgs::k_fft_plain<320, 16, -1, 3, 0>(cgs::PlainArgs)", "cgs::k_fft_plain<320, 16, -1, 4, 0>",
+             "cgs::k_fft_plain<320, 16, 1, 4, 0>", "cgs::k_fft_plain<320, 16, 1, 3, 0>",
+             "cgs::k_fft_zpre<224, 8, 256, true, true>", "cgs::k_fft_zpost<224, 8, 256>",
+             "cgs::k_grid<5, 4, 8, 8, 16, 2>", "cgs::k_degrid<5, 16>", "cgs::k_copy_rhs"]
+    got = [prof.stage_of(n, cnt) for n in names]
+    assert got == ["fft_y_fwd", "fft_x_fwd", "fft_x_inv", "fft_y_inv", "fft_z_fwd", "fft_z_inv", "grid", "degrid",
+                   None]
