@@ -1450,20 +1450,33 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     CKS(dalloc(h, &dv, (size_t)np));
     CKS(dalloc(h, &ds, (size_t)nc + 1));
     CKS(dalloc(h, &di, (size_t)np));
-    CKS(dalloc(h, &dov, 2));          // [0] cells to rebuild, [1] final overflows
+    int* redo2 = nullptr;
+    CKS(dalloc(h, &dov, 3));          // [0] cells to rebuild, [1] final overflows, [2] rebuilt again
     CKS(dalloc(h, &redo, (size_t)np));
+    CKS(dalloc(h, &redo2, (size_t)np));
     CK(hipMemcpy(dp, pts, sizeof(double) * 3 * np, hipMemcpyHostToDevice));
     CK(hipMemcpy(ds, start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
     CK(hipMemcpy(di, idx.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-    CK(hipMemset(dov, 0, 2 * sizeof(int)));
+    CK(hipMemset(dov, 0, 3 * sizeof(int)));
     H.start = ds;
     H.idx = di;
     CK(launch_voronoi(dp, np, H, dv, redo, dov, st));
-    int cnt[2] = {0, 0};
+    int cnt[3] = {0, 0, 0};
     CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
+    // Middle pass (medium limits in LDS, about five cells resident per CU) for
+    // short lists; its overflows go to the last pass.  Long lists (the long
+    // thin cells of sparse line sets) finish sooner in the last pass, which
+    // builds every listed cell at once from device memory.
+    if (cnt[0] > 0 && cnt[0] <= 6144) {
+        CK(launch_voronoi_mid(dp, H, redo, cnt[0], dv, redo2, dov + 2, st));
+        CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        std::swap(redo, redo2);
+        cnt[0] = cnt[2];
+    }
     if (cnt[0] > 0) {
-        // second pass: one thread per listed cell, each with a large cell
+        // last pass: one thread per listed cell, each with a large cell
         // workspace (about 330 KB) in device memory, as many as a quarter of
         // the free memory (at most 16 GB) holds
         size_t cell_b = 0, scr_b = 0;
@@ -1486,7 +1499,7 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     CK(hipMemcpyAsync(vol_host, dv, sizeof(double) * np, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
     dfree(h, dp, (size_t)np * 3); dfree(h, dv, (size_t)np); dfree(h, ds, (size_t)nc + 1);
-    dfree(h, di, (size_t)np); dfree(h, dov, 2); dfree(h, redo, (size_t)np);
+    dfree(h, di, (size_t)np); dfree(h, dov, 3); dfree(h, redo, (size_t)np); dfree(h, redo2, (size_t)np);
     if (over) return fail(h, CGS_ERR_ARG, std::to_string(over) + " Voronoi cells exceeded the face/vertex limits");
     return CGS_OK;
 }

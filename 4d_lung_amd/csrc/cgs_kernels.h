@@ -197,6 +197,8 @@ hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, dou
 // Second pass (large limits) over redo[0..nredo) with nthreads threads (a
 // multiple of 64), each using one cell and one scratch slot of the
 // workspaces; closed cells that still overflow are counted in *overflow.
+hipError_t launch_voronoi_mid(const double* pts, const VorHash& H, const int* redo, int nredo, double* vol,
+                              int* redo2, int* nredo2, hipStream_t s);
 void voronoi_large_sizes(size_t* cell, size_t* scratch);
 hipError_t launch_voronoi_large(const double* pts, const VorHash& H, const int* redo, int nredo, int nthreads,
                                 void* cells, void* scratch, double* vol, int* overflow, hipStream_t s);

@@ -1,5 +1,5 @@
 // cgs_voronoi.hip -- Voronoi density compensation (estimateDensityV3D,
-// functions.h:161-258) by half-space clipping, one cell per thread.
+// functions.h:161-258) by half-space clipping, one cell per wave (first pass).
 //
 // The reference takes each sample's Voronoi cell volume from a CGAL Delaunay
 // triangulation (sum over Delaunay edges of dual-facet area * edge length / 6,
@@ -13,8 +13,8 @@
 // Voronoi region reaches more than one hash cell beyond the point cloud.  The
 // volume is the reference's face sum area * |q - p| / 6 with areas as
 // triangle fans from the first vertex.  A cell is a pool of vertices and
-// faces as rings of pool indices (about 12.7 KB per thread with the
-// first-pass limits).
+// faces as rings of pool indices (about 12.7 KB with the first-pass limits,
+// held in LDS).
 //
 // Built with -ffp-contract=off; every operation is a separately rounded
 // IEEE double operation in a fixed order, so the weights are reproducible
@@ -40,6 +40,7 @@ struct Limits {
     using Idx = I;
 };
 using SmallLimits = Limits<64, 48, 128, uint8_t>;
+using MediumLimits = Limits<128, 96, 254, uint8_t>;
 using LargeLimits = Limits<384, 256, 2048, uint16_t>;
 
 struct V3 {
@@ -334,22 +335,42 @@ __device__ double point_volume(const double* __restrict__ p, long long i, const 
     return cell_volume(c);
 }
 
-// First pass: one cell per thread with the small limits in private memory,
-// grid-stride over the points; cells that outgrow them are listed for the
-// second pass.
+// First pass: one cell per workgroup of one wave, built by its first lane in
+// LDS (about 12.7 KB with the small limits, so about twelve cells are built
+// at once on a CU); the other lanes idle.  The clipping is a long chain of
+// dependent, data-dependent accesses to the cell, which LDS serves far faster
+// than private memory.  Grid-stride over the points; cells that outgrow the
+// limits are listed for the second pass.
 __global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, long long np, VorHash H,
                                                double* __restrict__ vol, int* __restrict__ redo,
                                                int* __restrict__ nredo) {
-    Cell<SmallLimits> c;
-    ClipScratch<SmallLimits> w;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < np;
-         i += (long long)gridDim.x * blockDim.x) {
+    __shared__ Cell<SmallLimits> c;
+    __shared__ ClipScratch<SmallLimits> w;
+    if (threadIdx.x != 0) return;
+    for (long long i = blockIdx.x; i < np; i += gridDim.x) {
         vol[i] = point_volume(p, i, H, c, w, false);
         if (c.overflow) redo[atomicAdd(nredo, 1)] = (int)i;
     }
 }
 
-// Second pass over the listed points with the large limits; each thread's
+// Middle pass over the listed points with the medium limits, again one cell
+// per wave in LDS (about 32 KB); cells that outgrow these too are listed for
+// the last pass.
+__global__ __launch_bounds__(64) void k_voronoi_mid(const double* __restrict__ p, VorHash H,
+                                                   const int* __restrict__ redo, int nredo,
+                                                   double* __restrict__ vol, int* __restrict__ redo2,
+                                                   int* __restrict__ nredo2) {
+    __shared__ Cell<MediumLimits> c;
+    __shared__ ClipScratch<MediumLimits> w;
+    if (threadIdx.x != 0) return;
+    for (long long k = blockIdx.x; k < nredo; k += gridDim.x) {
+        const long long i = redo[k];
+        vol[i] = point_volume(p, i, H, c, w, false);
+        if (c.overflow) redo2[atomicAdd(nredo2, 1)] = (int)i;
+    }
+}
+
+// Last pass over the listed points with the large limits; each thread's
 // cell and working arrays live in a device workspace (too large for private
 // memory).  Closed cells that outgrow these limits as well are counted.
 __global__ __launch_bounds__(64) void k_voronoi_large(const double* __restrict__ p, VorHash H,
@@ -371,13 +392,18 @@ __global__ __launch_bounds__(64) void k_voronoi_large(const double* __restrict__
 
 hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* redo, int* nredo,
                           hipStream_t s) {
-    // Each thread holds a small-limit cell of about 12.7 KB in private
-    // (scratch) memory, which the runtime backs for every wave of the launch.
-    // The grid is capped at two blocks per CU (512 waves, about 420 MB of
-    // scratch) and 64 KB of dynamic LDS per block keeps at most two of them
-    // resident on a CU.
-    const long long blocks = std::min<long long>((np + 63) / 64, 512);
-    vor::k_voronoi<<<(unsigned)blocks, 64, 65536, s>>>(pts, np, H, vol, redo, nredo);
+    // one wave per cell in flight; 8192 workgroups cover every CU several
+    // times over and grid-stride over the rest
+    const long long blocks = std::min<long long>(np, 8192);
+    if (blocks == 0) return hipSuccess;
+    vor::k_voronoi<<<(unsigned)blocks, 64, 0, s>>>(pts, np, H, vol, redo, nredo);
+    return hipGetLastError();
+}
+
+hipError_t launch_voronoi_mid(const double* pts, const VorHash& H, const int* redo, int nredo, double* vol,
+                              int* redo2, int* nredo2, hipStream_t s) {
+    if (nredo == 0) return hipSuccess;
+    vor::k_voronoi_mid<<<(unsigned)std::min(nredo, 8192), 64, 0, s>>>(pts, H, redo, nredo, vol, redo2, nredo2);
     return hipGetLastError();
 }
 
