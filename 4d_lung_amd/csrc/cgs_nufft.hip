@@ -56,6 +56,14 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
         iy[o] = min(max(bs.y + o, 0), a.g.ny - 1);
         iz[o] = min(max(bs.z + o, 0), a.g.nz - 1);
     }
+    // Rows (columns) whose y (z) factor is 0 give weight 0 to every tap: their
+    // loads are pointed at the centre row (column), whose lines the lanes load
+    // anyway, so they cost no L2 request (a finite value times 0 adds nothing).
+#pragma unroll
+    for (int o = 0; o < KW; ++o) {
+        if (vy[o] == 0.0) iy[o] = iy[KW / 2];
+        if (vz[o] == 0.0) iz[o] = iz[KW / 2];
+    }
     const int C = a.g.C;
     const float2* __restrict__ G = a.G + (long long)(bin - a.b0) * a.g.bp;
     const long long ys = (long long)a.g.nz * C;
