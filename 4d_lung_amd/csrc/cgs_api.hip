@@ -32,6 +32,7 @@ struct BinData {
     float2* y = nullptr;         // [m][C] measured data (no density weights), or null
     float* dcf_full = nullptr;   // [nS] own density weights, or null (context's are used)
     bool built = false;
+    bool has_y = false;          // values set (also for a bin without samples, m = 0)
 };
 
 struct ProfEntry {
@@ -358,10 +359,12 @@ void free_bin(cgs_ctx* h, BinData& b) {
     dfree(h, b.y, b.m * h->C);
     b.m = 0;
     b.built = false;
+    b.has_y = false;
 }
 
 void free_packed(cgs_ctx* h) {
-    const long long mt = h->bin_off.empty() ? 0 : h->bin_off.back();
+    // packed arrays hold at least one element (bins without samples)
+    const long long mt = std::max(1LL, h->bin_off.empty() ? 0 : h->bin_off.back());
     if (!h->p_alias) {
         dfree(h, h->p_base, mt);
         dfree(h, h->p_vals, mt * val_stride(h->kb.kw));
@@ -517,6 +520,7 @@ int gather_bin_data(cgs_ctx* h, int bin, const float2* d_data) {
     dfree(h, B.y, B.m * h->C);
     CKS(dalloc(h, &B.y, (size_t)B.m * h->C));
     CK(launch_gather_data(d_data, h->ns, B.base, B.m, h->C, nullptr, nullptr, nullptr, B.y, h->st));
+    B.has_y = true;
     return CGS_OK;
 }
 
@@ -531,18 +535,19 @@ int pack_bins(cgs_ctx* h) {
     h->bin_off.assign(h->nbins + 1, 0);
     for (int b = 0; b < h->nbins; ++b) h->bin_off[b + 1] = h->bin_off[b] + h->bins[b].m;
     const long long mt = h->bin_off.back();
+    const long long mt1 = std::max(1LL, mt);   // a bin may have no samples
     bool all_y = true;
-    for (int b = 0; b < h->nbins; ++b) all_y = all_y && h->bins[b].y != nullptr;
-    if (h->nbins == 1) {
+    for (int b = 0; b < h->nbins; ++b) all_y = all_y && h->bins[b].has_y;
+    if (h->nbins == 1 && mt > 0) {
         h->p_alias = true;
         h->p_base = h->bins[0].base;
         h->p_vals = h->bins[0].vals;
         h->p_y = h->bins[0].y;
         h->p_tile_off = h->bins[0].tile_off;
     } else {
-        CKS(dalloc(h, &h->p_base, mt));
-        CKS(dalloc(h, &h->p_vals, (size_t)mt * vs));
-        if (all_y) CKS(dalloc(h, &h->p_y, (size_t)mt * h->C));
+        CKS(dalloc(h, &h->p_base, mt1));
+        CKS(dalloc(h, &h->p_vals, (size_t)mt1 * vs));
+        if (all_y) CKS(dalloc(h, &h->p_y, (size_t)mt1 * h->C));
         CKS(dalloc(h, &h->p_tile_off, (size_t)h->nbins * (nt + 1)));
         std::vector<int> off(nt + 1);
         for (int b = 0; b < h->nbins; ++b) {
@@ -1160,6 +1165,7 @@ int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, co
     dfree(h, B.y, B.m * h->C);
     CKS(dalloc(h, &B.y, (size_t)B.m * h->C));
     if (B.m) CK(hipMemcpy(B.y, ys.data(), sizeof(float2) * B.m * h->C, hipMemcpyHostToDevice));
+    B.has_y = true;
     dfree(h, B.dcf_full, h->ns);
     if (dcf) {
         std::vector<float> full(h->ns, 0.0f);
@@ -1175,7 +1181,7 @@ int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter) {
     CKS(check_ready(h, true));
     CKS(pack_bins(h));
     for (int b = 0; b < h->nbins; ++b)
-        if (!h->bins[b].y) return fail(h, CGS_ERR_STATE, "bin " + std::to_string(b) + " has no data");
+        if (!h->bins[b].has_y) return fail(h, CGS_ERR_STATE, "bin " + std::to_string(b) + " has no data");
     if (!h->p_y) return fail(h, CGS_ERR_STATE, "bin data not packed");
     CKS(ensure_vectors(h));
     CKS(choose_group(h));

@@ -239,6 +239,58 @@ def test_batched_bins_match_plain_and_oracle(pkg, orc):
     o.exclude_missing_data(P["data"])
 
 
+def test_empty_bin_follows_the_reference(pkg, orc):
+    """A bin without acquired samples between two ordinary bins.  In
+    Reco::cgSenseDynamic its right-hand side is 0, so rr = pq = 0 and
+    complexDivision gives alpha = 0/0 (Reco.cpp:145-151, 276-285): the first
+    pass leaves a NaN image and a NaN residual, and `NaN > threshold` ends the
+    loop after that pass (Reco.cpp:230).  The oracle does the same; the GPU
+    path must too, in both solve modes, for an empty bin given as zero-filled
+    data and as an empty compact sample list (with its Voronoi weights), and
+    the other bins of the launch must equal the oracle's solves."""
+    P = small_problem(orc, seed=41)
+    o = P["oracle"]
+    lines = P["lines"]
+    acq = np.flatnonzero(lines.ravel())
+    labels = np.full(lines.size, -1)
+    labels[acq] = np.where(np.arange(acq.size) % 2 == 0, 0, 2)
+    labels = labels.reshape(lines.shape)
+    datas = []
+    for b in range(3):
+        sm = np.broadcast_to((labels == b)[None], (P["samp"][0],) + lines.shape).reshape(-1)
+        datas.append(np.where(sm[None, :], P["data"], 0).astype(np.complex64))
+    assert not datas[1].any()
+    refs = []
+    for b in range(3):
+        o.calculate_weights(datas[b])
+        refs.append(o.cg_sense(datas[b], thresh=0.0, max_iter=6, prec=64))
+    o.calculate_weights()
+    o.exclude_missing_data(P["data"])
+    xr1, rr1, itr1 = refs[1]
+    assert itr1 == 1 and np.isnan(xr1).all() and np.isnan(rr1[0])
+
+    ctx = gpu_context(pkg, P)
+    ctx.set_num_bins(3)
+    for compact in (False, True):
+        for b in range(3):
+            if compact:
+                idx = np.flatnonzero(datas[b][0] != 0).astype(np.int32)
+                ctx.set_bin_samples(b, idx, datas[b][:, idx])
+            else:
+                ctx.set_bin_data(b, datas[b])
+        if compact:
+            assert ctx.bin_count(1) == 0
+            ctx.set_bin_voronoi(1)
+        for mode in (1, 0):
+            print("empty bin: compact=%s mode=%d" % (compact, mode), flush=True)
+            x, res, it = ctx.cg_sense_bins(mode=mode, thresh=0.0, max_iter=6)
+            assert it[1] == 1 and np.isnan(x[1]).all() and np.isnan(res[1, 0])
+            for b in (0, 2):
+                xr, _, itr = refs[b]
+                assert it[b] == itr == 7
+                assert rel_l2(x[b], xr) < CG_TOL
+
+
 def test_compact_bin_samples_equal_zero_filled(pkg, orc, small):
     P = small
     ctx = gpu_context(pkg, P)

@@ -338,3 +338,22 @@ def test_exclude_missing_data_drops_zero_samples(orc):
     counts, _, _ = P["oracle"].get_weights()
     assert np.all(counts[P["data"][0] == 0] == 0)
     assert np.all(counts[P["data"][0] != 0] > 0)
+
+
+def test_cg_on_empty_data_makes_one_nan_pass(orc):
+    """Reco::cgSenseDynamic on a dataset without acquired samples: the
+    right-hand side is 0, so rr = pq = 0 and complexDivision's 0/0
+    (Reco.cpp:145-151) makes alpha NaN; x and r become NaN after the first
+    pass and `NaN > threshold` is false, so the loop stops there
+    (Reco.cpp:230, 276-309).  Both precisions of the oracle follow it."""
+    from problems import small_problem
+
+    P = small_problem(orc, seed=41)
+    o = P["oracle"]
+    d = np.zeros_like(P["data"])
+    o.calculate_weights(d)
+    counts, _, _ = o.get_weights()
+    assert not np.any(counts)
+    for prec in (64, 32):
+        x, res, it = o.cg_sense(d, thresh=0.0, max_iter=6, prec=prec)
+        assert it == 1 and res.shape == (1,) and np.isnan(res[0]) and np.isnan(x).all()
