@@ -8,4 +8,4 @@ that ABI, ``synth`` the seeded synthetic workload generator and
 of E / E^H with coordinate gradients and the unrolled CG-SENSE.
 """
 from . import cgs, synth  # noqa: F401
-from .cgs import CgsError, Context, Fourier, SOLVE_BATCHED, SOLVE_PLAIN  # noqa: F401
+from .cgs import CGS_ERR_NONFINITE, CgsError, Context, Fourier, SOLVE_BATCHED, SOLVE_PLAIN  # noqa: F401

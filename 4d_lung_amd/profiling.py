@@ -11,8 +11,9 @@ scaled by what they read for the calibration copy (1 GiB read and 1 GiB
 written by the runtime's copy kernel).  Per stage the median over the run's
 launches is reported (setup launches of the same kernels are outnumbered by
 the CG iterations).  Writes
-``OUT_DIR/kernels_WORKLOAD.md`` (durations) and ``OUT_DIR/traffic_WORKLOAD.txt``
-(bytes per launch, read back by bench.py for ``roofline.traffic``).  With
+``OUT_DIR/kernels_WORKLOAD.md`` (durations; a description above the tables of
+an existing file is kept) and ``OUT_DIR/traffic_WORKLOAD.txt`` (bytes per
+launch, read back by bench.py for ``roofline.traffic``).  With
 BINS_PER_LAUNCH (binned workloads profiled with every launch covering that
 many bins) the traffic is written per bin.
 """
@@ -24,6 +25,7 @@ import statistics
 import sys
 
 CAL_BYTES = 1 << 30
+TABLE_MARK = "Per CG stage (kernel trace):"
 
 
 def _short(name):
@@ -108,8 +110,14 @@ def main(argv):
                 fb = med(fetch[st]) / nb
                 wb = med(write[st]) / nb
                 f.write("%s %.6g %.6g\n" % (st, fb, wb))
-    with open(os.path.join(out_dir, "kernels_%s.md" % workload), "w") as f:
-        f.write("Per CG stage (kernel trace):\n\n")
+    kpath = os.path.join(out_dir, "kernels_%s.md" % workload)
+    head = ""
+    if os.path.exists(kpath):   # the description above the tables is kept
+        with open(kpath) as f:
+            head = f.read().split(TABLE_MARK)[0]
+    with open(kpath, "w") as f:
+        f.write(head)
+        f.write(TABLE_MARK + "\n\n")
         f.write("| stage | kernel | launches | median us | min us | max us |\n|---|---|---|---|---|---|\n")
         for st in order:
             d = dur[st]

@@ -81,3 +81,33 @@ def test_stage_names_of_kernel_trace():
     got = [prof.stage_of(n, cnt) for n in names]
     assert got == ["fft_y_fwd", "fft_x_fwd", "fft_x_inv", "fft_y_inv", "fft_z_fwd", "fft_z_inv", "grid", "degrid",
                    None]
+
+
+def test_kernel_summary_keeps_its_description(tmp_path):
+    """Rewriting kernels_<workload>.md from a new trace keeps the text above
+    the tables (workload description and commands) and replaces the tables."""
+    tr = tmp_path / "trace"
+    tr.mkdir()
+    (tr / "run_kernel_trace.csv").write_text(
+        "Kernel_Name,Start_Timestamp,End_Timestamp\n"
+        "\"cgs::k_grid<5, 4, 8, 8, 16, 2>(cgs::GridArgs)\",0,1000\n"
+        "\"cgs::k_degrid<5, 16>(cgs::GridArgs)\",2000,2500\n")
+    for d, c in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
+        (tmp_path / d).mkdir()
+        (tmp_path / d / "run_counter_collection.csv").write_text(
+            "Kernel_Name,Counter_Name,Counter_Value\n"
+            "__amd_rocclr_copyBuffer,%s,1048576\n"
+            "\"cgs::k_grid<5, 4, 8, 8, 16, 2>(cgs::GridArgs)\",%s,100\n"
+            "\"cgs::k_degrid<5, 16>(cgs::GridArgs)\",%s,200\n" % (c, c, c))
+    out = tmp_path / "out"
+    out.mkdir()
+    md = out / "kernels_w.md"
+    md.write_text("# title\n\nworkload text\n\n" + prof.TABLE_MARK + "\n\n| old table |\n")
+    args = ["x", str(tr), str(tmp_path / "fetch"), str(tmp_path / "write"), "w", str(out)]
+    assert prof.main(args) == 0
+    text = md.read_text()
+    assert text.startswith("# title\n\nworkload text\n\n" + prof.TABLE_MARK)
+    assert "old table" not in text and "`cgs::k_grid<5, 4, 8, 8, 16, 2>`" in text
+    assert text.count(prof.TABLE_MARK) == 1
+    t, _ = prof.read_traffic(str(out / "traffic_w.txt"))
+    assert set(t) == {"grid", "degrid"}

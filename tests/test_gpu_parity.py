@@ -291,6 +291,34 @@ def test_empty_bin_follows_the_reference(pkg, orc):
                 assert rel_l2(x[b], xr) < CG_TOL
 
 
+def test_nonfinite_data_is_a_status_not_an_abort(pkg, orc):
+    """One Inf among the measured samples spreads through E^H and the FFT, so
+    q[0] of the first pass is not finite: the reference prints an error and
+    exits (Reco.cpp:264-268), the oracle returns its error status, and the
+    C-ABI returns CGS_ERR_NONFINITE with a message in both solve modes.  The
+    context stays usable: it then solves the clean data like the oracle."""
+    P = small_problem(orc, seed=51)
+    o = P["oracle"]
+    acq = np.flatnonzero(P["data"][0] != 0)
+    bad = P["data"].copy()
+    bad[0, acq[acq.size // 2]] = np.complex64(np.inf)
+    with pytest.raises(RuntimeError, match="status 3"):
+        o.cg_sense(bad, thresh=0.0, max_iter=4, prec=64)
+    ctx = gpu_context(pkg, P)
+    for mode in (1, 0):
+        print("non-finite data: mode=%d" % mode, flush=True)
+        ctx.set_bin_data(0, bad)
+        with pytest.raises(pkg.CgsError) as e:
+            ctx.cg_sense_bins(mode=mode, thresh=0.0, max_iter=4)
+        assert e.value.status == pkg.CGS_ERR_NONFINITE
+        assert "NaN or INF" in str(e.value)
+    ctx.set_bin_data(0, P["data"])
+    x, _, it = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=4)
+    xr, _, itr = o.cg_sense(P["data"], thresh=0.0, max_iter=4, prec=64)
+    assert it[0] == itr == 5
+    assert rel_l2(x[0], xr) < CG_TOL
+
+
 def test_compact_bin_samples_equal_zero_filled(pkg, orc, small):
     P = small
     ctx = gpu_context(pkg, P)
