@@ -155,6 +155,34 @@ def test_cg_bins_matches_oracle(pkg, orc, small_dcf, mode):
     assert rel_l2(x[0], xr) < CG_TOL
 
 
+def test_graph_replays_between_eager_work(pkg, orc, small_dcf):
+    """The captured CG iteration replayed in several calls with eager work in
+    between (result readback, plain-mode iterations on the same state) gives
+    after every call the same image and residuals as a solve that never uses
+    the graph, and the final image matches the oracle."""
+    P = small_dcf
+    ctx = gpu_context(pkg, P)
+    ctx.set_bin_data(0, P["data"], P["dcf"])
+    max_iter = 9
+    ref = []
+    ctx.cg_prepare(0.0, max_iter)
+    for n in (2, 3, 1, 4):
+        ctx.cg_iterate(0, n)
+        ref.append(ctx.cg_results())
+    ctx.cg_prepare(0.0, max_iter)
+    for k, (mode, n) in enumerate(((1, 2), (1, 3), (0, 1), (1, 4))):
+        print("graph replays: call %d mode %d x %d" % (k, mode, n), flush=True)
+        ctx.cg_iterate(mode, n)
+        x, res, its = ctx.cg_results()
+        xe, rese, itse = ref[k]
+        assert its[0] == itse[0]
+        assert rel_l2(x[0], xe[0]) < 1e-6
+        np.testing.assert_allclose(res, rese, rtol=1e-6)
+    assert its[0] == max_iter + 1
+    xr, _, _ = P["oracle"].cg_sense(P["data"], thresh=0.0, max_iter=max_iter, prec=64)
+    assert rel_l2(x[0], xr) < CG_TOL
+
+
 def test_cg_threshold_stops_like_reference(pkg, orc, small):
     """Loop control of Reco.cpp:230, 308-309 with an absolute threshold."""
     P = small
