@@ -111,3 +111,15 @@ def test_kernel_summary_keeps_its_description(tmp_path):
     assert text.count(prof.TABLE_MARK) == 1
     t, _ = prof.read_traffic(str(out / "traffic_w.txt"))
     assert set(t) == {"grid", "degrid"}
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The benchmark measures one GPU; asked for more it exits with a message
+    before touching a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
