@@ -494,6 +494,9 @@ def main():
                      "traffic": traffic.get(dom, (None, None))[0] + traffic[dom][1] if dom in traffic else None,
                      "traffic_source": traffic_src if dom in traffic else None,
                      "algorithmic_bytes": dk["bytes"], "avg_ms": dk["avg_ms"]},
+        # every stage runs once per iteration: the iteration's algorithmic
+        # bytes times the iteration rate
+        "algorithmic_gbs_per_iteration": sum(v["bytes"] or 0 for v in kernels.values()) * value / 1e9,
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "gbs": round(v["gbs"], 1) if v["gbs"] else None}
                     for k, v in kernels.items()},
     }
