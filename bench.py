@@ -129,6 +129,8 @@ def bench_binned(args, cfg, pkg, torch):
     rho = synth.phantom(grid, image, cfg["seed"] + 2).reshape(-1)
     log("synthetic image and maps %.1fs (%d lines in %d bins)" % (time.time() - t0, int(lines.sum()), B))
     ctx = pkg.Context(grid, samp, C)
+    if args.pruning >= 0:
+        ctx.set_pruning(args.pruning)
     ctx.set_trajectory(traj)
     ctx.calculate_kernel_ft()
     ctx.set_maps(maps)
@@ -336,6 +338,8 @@ def main():
                     help="binned workloads: iterations of the oracle check (default: the workload's own, else "
                          "--steps); with fewer than --steps the checked bins are solved again for that many")
     ap.add_argument("--bin-group", type=int, default=-1, help="binned workloads: bins per launch (-1: autotune)")
+    ap.add_argument("--pruning", type=int, default=-1, choices=(-1, 0, 1, 2),
+                    help="column pruning of the x passes: 0 off, 1 automatic, 2 on (-1: the library's default)")
     ap.add_argument("--no-plain", action="store_true", help="binned workloads: skip timing the plain path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (the reference kernel that scales the "
@@ -380,6 +384,8 @@ def main():
     log("synthetic inputs %.1fs (%d lines acquired of %d)" % (time.time() - t0, lines.sum(), lines.size))
 
     ctx = pkg.Context(grid, samp, C)
+    if args.pruning >= 0:
+        ctx.set_pruning(args.pruning)
     ctx.set_trajectory(traj)
     ctx.calculate_kernel_ft()
     ctx.set_maps(maps)
