@@ -472,3 +472,24 @@ def test_column_pruning_keeps_images(pkg, orc):
     assert rel_l2(out[2][3], o.nufft_bwd(d0.astype(np.complex128), prec=64)) < OP_TOL
     o.calculate_weights()
     o.exclude_missing_data(P["data"])
+
+
+@pytest.mark.parametrize("ncoil", [1, 5, 12, 20])
+def test_coil_counts_off_the_tile_widths(pkg, orc, ncoil):
+    """Coil counts that are not multiples of the kernels' coil tiles (one
+    coil; 5 and 12: padded to 8 / 16 coils in gridding and a short last
+    8-coil z-pass chunk; 20: padded to 32 with chunks of 8, 8, 4): E^H and E
+    against the FP64 oracle, and the batched CG-SENSE solve."""
+    P = small_problem(orc, ncoil=ncoil, seed=70 + ncoil)
+    o = P["oracle"]
+    ctx = gpu_context(pkg, P)
+    print("coils=%d: E^H, E" % ncoil, flush=True)
+    assert rel_l2(ctx.nufft_bwd(P["data"]), o.nufft_bwd(P["data"], prec=64)) < OP_TOL
+    img = (P["maps"] * P["rho"][None, :]).astype(np.complex64)
+    assert rel_l2(ctx.nufft_fwd(img), o.nufft_fwd(img, prec=64)) < OP_TOL
+    print("coils=%d: CG" % ncoil, flush=True)
+    ctx.set_bin_data(0, P["data"])
+    x, _, its = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=7)
+    xr, _, itr = o.cg_sense(P["data"], thresh=0.0, max_iter=7, prec=64)
+    assert its[0] == itr == 8
+    assert rel_l2(x[0], xr) < CG_TOL
