@@ -493,3 +493,49 @@ def test_coil_counts_off_the_tile_widths(pkg, orc, ncoil):
     xr, _, itr = o.cg_sense(P["data"], thresh=0.0, max_iter=7, prec=64)
     assert its[0] == itr == 8
     assert rel_l2(x[0], xr) < CG_TOL
+
+
+@pytest.mark.parametrize("w,b,kw", [(2.5, 5.2, 3), (6.0, 13.9, 7)])
+def test_other_kernel_widths(pkg, orc, small, w, b, kw):
+    """ConvKernel::setW / setB (Reco.h:45-56) with a 3- and a 7-point window
+    per axis: kernelFT bit-exact; E^H, E and the CG-SENSE solve against the
+    FP64 oracle built with the same (w, b).  The bar is the reference's own
+    FP32 arithmetic (the oracle's FP32 restatement of Reco.cpp) on the same
+    inputs: the GPU may be at most 3x as far from FP64 as that, and never
+    further than the default bars.  It matters for the 7-point kernel: its
+    kernelFT spans a factor of about 3000, the deapodization magnifies FP32
+    rounding at the grid edge, and the reference arithmetic itself lands
+    7e-6 (E^H) and 1.1e-4 (8 CG passes) from FP64 there."""
+    P = small
+    o = orc.OracleReco(P["grid"], P["samp"], P["ncoil"], w=w, b=b)
+    o.set_trajectory(P["traj"])
+    o.calculate_weights()
+    o.calculate_kernel_ft()
+    o.set_maps(P["maps"])
+    o.exclude_missing_data(P["data"])
+    ctx = pkg.Context(P["grid"], P["samp"], P["ncoil"])
+    ctx.set_kernel(w, b)
+    ctx.set_trajectory(P["traj"])
+    ctx.calculate_weights()
+    ctx.exclude_missing_data(P["data"])
+    ctx.calculate_kernel_ft()
+    ctx.set_maps(P["maps"])
+    assert ctx.kernel_info()[0] == kw
+    print("kernel width %.2f: %d-point window" % (w, kw), flush=True)
+    np.testing.assert_array_equal(ctx.kernel_ft(), o.get_kernel_ft())
+
+    def bar(default, ref32, ref64):
+        return max(default, 3.0 * rel_l2(ref32, ref64))
+
+    eh64 = o.nufft_bwd(P["data"].astype(np.complex128), prec=64)
+    eh = ctx.nufft_bwd(P["data"])
+    assert rel_l2(eh, eh64) < bar(OP_TOL, o.nufft_bwd(P["data"], prec=32), eh64)
+    img = (P["maps"] * P["rho"][None, :]).astype(np.complex64)
+    e64 = o.nufft_fwd(img.astype(np.complex128), prec=64)
+    assert rel_l2(ctx.nufft_fwd(img), e64) < bar(OP_TOL, o.nufft_fwd(img, prec=32), e64)
+    ctx.set_bin_data(0, P["data"])
+    x, _, its = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=7)
+    xr, _, itr = o.cg_sense(P["data"], thresh=0.0, max_iter=7, prec=64)
+    x32, _, _ = o.cg_sense(P["data"], thresh=0.0, max_iter=7, prec=32)
+    assert its[0] == itr == 8
+    assert rel_l2(x[0], xr) < bar(CG_TOL, x32, xr)
