@@ -74,7 +74,10 @@ def lib():
         L.orc_voronoi_volumes.restype = ctypes.c_long
         L.orc_voronoi_volumes.argtypes = [P, ctypes.c_long, P, ctypes.c_int]
         L.orc_voronoi_density.restype = ctypes.c_long
-        L.orc_voronoi_density.argtypes = [P, ctypes.c_long, P, ctypes.c_int]
+        L.orc_voronoi_density.argtypes = [P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_double, P,
+                                          ctypes.c_int]
+        L.orc_density_from_volumes.restype = None
+        L.orc_density_from_volumes.argtypes = [P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_double, P]
         _lib = L
     return _lib
 
@@ -109,12 +112,23 @@ def voronoi_volumes(points, nthreads=0):
     return vol
 
 
-def voronoi_density(points, nthreads=0):
+def voronoi_density(points, average=False, sx=0, cutoff=1.0, nthreads=0):
+    """estimateDensityV3D (functions.h:161-258) with the clipped volumes."""
     p = np.ascontiguousarray(points, dtype=np.float64)
     d = np.zeros(len(p), np.float32)
-    over = lib().orc_voronoi_density(_ptr(p), len(p), _ptr(d), int(nthreads))
+    over = lib().orc_voronoi_density(_ptr(p), len(p), int(bool(average)), int(sx), float(cutoff), _ptr(d),
+                                     int(nthreads))
     if over:
         raise RuntimeError("%d Voronoi cells exceeded the face/vertex limits" % over)
+    return d
+
+
+def density_from_volumes(vol, average=False, sx=0, cutoff=1.0):
+    """The averaging and normalisation of estimateDensityV3D
+    (functions.h:212-246) applied to given cell volumes."""
+    v = np.ascontiguousarray(vol, dtype=np.float64)
+    d = np.zeros(len(v), np.float32)
+    lib().orc_density_from_volumes(_ptr(v), len(v), int(bool(average)), int(sx), float(cutoff), _ptr(d))
     return d
 
 

@@ -416,18 +416,38 @@ long orc_voronoi_volumes(const double* pts, long np, double* vol, int nthreads) 
     return over;
 }
 
-// estimateDensityV3D without averaging (functions.h:234-245): float volumes
-// divided by their maximum.
-long orc_voronoi_density(const double* pts, long np, float* dcf, int nthreads) {
-    std::vector<double> vol(np);
-    const long over = orc_voronoi_volumes(pts, np, vol.data(), nthreads);
-    double mx = 0.0;
-    for (long i = 0; i < np; i++) {
-        dcf[i] = (float)vol[i];
-        if (dcf[i] > mx) mx = dcf[i];
+// The tail of estimateDensityV3D (functions.h:212-246) on given volumes:
+// float cell volumes; with average, per-class float sums over i % sX (the
+// reference sums them in a parallel loop with a data race; here in index
+// order), their double total, and every volume replaced by
+// total^-1 * sum[min(i % sX, (int)((sX - 1) * cutoff))]; finally the volumes
+// divided by their maximum (double reciprocal, float result).
+void orc_density_from_volumes(const double* vol, long np, int average, int sx, double cutoff, float* dcf) {
+    for (long i = 0; i < np; i++) dcf[i] = (float)vol[i];
+    if (average && sx > 0) {
+        std::vector<float> sum(sx, 0.0f);
+        for (long i = 0; i < np; i++) sum[i % sx] += dcf[i];
+        double tot = 0.0;
+        for (int k = 0; k < sx; k++) tot += sum[k];
+        tot = 1.0 / tot;
+        const int cut = (int)((sx - 1) * cutoff);
+        for (long i = 0; i < np; i++) {
+            const int k = (int)(i % sx) < cut ? (int)(i % sx) : cut;
+            dcf[i] = (float)(tot * sum[k]);
+        }
     }
+    double mx = 0.0;
+    for (long i = 0; i < np; i++)
+        if (dcf[i] > mx) mx = dcf[i];
     const double inv = 1.0 / mx;
     for (long i = 0; i < np; i++) dcf[i] = (float)((double)dcf[i] * inv);
+}
+
+// estimateDensityV3D: clipped cell volumes, then the tail above.
+long orc_voronoi_density(const double* pts, long np, int average, int sx, double cutoff, float* dcf, int nthreads) {
+    std::vector<double> vol(np);
+    const long over = orc_voronoi_volumes(pts, np, vol.data(), nthreads);
+    orc_density_from_volumes(vol.data(), np, average, sx, cutoff, dcf);
     return over;
 }
 

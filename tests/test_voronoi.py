@@ -140,3 +140,40 @@ def test_bin_voronoi_dcf_in_cg(pkg, orc):
     xr, _, _ = P["oracle"].cg_sense(P["data"], thresh=0.0, max_iter=7, prec=64)
     P["oracle"].set_density(None)
     assert rel_l2(x[0], xr) < 1e-4
+
+
+def test_density_averaging_closed_form(orc):
+    """The averaging and normalisation of estimateDensityV3D
+    (functions.h:212-246) on volumes exact in float: every sample of class
+    k = i % sX has volume k + 1, so the class sums are n_k (k + 1) exactly;
+    each sample takes its class's sum (classes above (sX - 1) * cutoff take
+    the cutoff class's), over the total, and after the division by the
+    maximum the weights are s[k'] / max s[k'] up to float rounding.  Without
+    averaging the weights are the volumes over their maximum."""
+    sx = 8
+    i = np.arange(8 * 37 + 5)                 # ragged: the first 5 classes have one sample more
+    k = i % sx
+    vol = (k + 1).astype(np.float64)
+    s = np.bincount(k, minlength=sx) * (np.arange(sx) + 1.0)
+    for cutoff in (1.0, 0.5):
+        kk = np.minimum(k, int((sx - 1) * cutoff))
+        d = orc.density_from_volumes(vol, average=True, sx=sx, cutoff=cutoff)
+        np.testing.assert_allclose(d, s[kk] / s[kk].max(), rtol=3e-7)
+        assert d.max() == 1.0
+    d = orc.density_from_volumes(vol)
+    np.testing.assert_allclose(d, vol / vol.max(), rtol=1e-7)
+    assert d.max() == 1.0
+
+
+@pytest.mark.gpu
+def test_device_density_averaging_bit_exact(pkg, orc):
+    """estimateDensityV3D with per-readout averaging on the device
+    (cgs_voronoi_density) equals the oracle's statement bit for bit, on every
+    point of a wave-CAIPI trajectory (the reference's own input: all lines)."""
+    grid, samp = (64, 64, 48), (64, 32, 24)
+    p = synth.wave_caipi_trajectory(grid, samp, 3, 4.0)
+    vol = orc.voronoi_volumes(p)
+    for cutoff in (1.0, 0.75):
+        print("density averaging: cutoff %.2f" % cutoff, flush=True)
+        got = pkg.cgs.voronoi_density(p, average=True, sx=samp[0], cutoff=cutoff)
+        np.testing.assert_array_equal(got, orc.density_from_volumes(vol, True, samp[0], cutoff))
