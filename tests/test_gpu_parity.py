@@ -539,3 +539,49 @@ def test_other_kernel_widths(pkg, orc, small, w, b, kw):
     x32, _, _ = o.cg_sense(P["data"], thresh=0.0, max_iter=7, prec=32)
     assert its[0] == itr == 8
     assert rel_l2(x[0], xr) < bar(CG_TOL, x32, xr)
+
+
+def test_autotuner_leaves_a_consistent_context(pkg, orc):
+    """tune_tile (every gridding tile shape, bins re-sorted after each change)
+    and tune_group (bins per batched launch) on five ragged bins: the context
+    they leave solves every bin like the oracle."""
+    import importlib
+
+    tune = importlib.import_module("4d_lung_amd.autotune")
+    P = small_problem(orc, seed=81)
+    o = P["oracle"]
+    lines = P["lines"]
+    acq = np.flatnonzero(lines.ravel())
+    labels = np.full(lines.size, -1)
+    labels[acq] = np.random.default_rng(82).choice(5, acq.size, p=[0.4, 0.25, 0.15, 0.12, 0.08])
+    labels = labels.reshape(lines.shape)
+    bins = []
+    for b in range(5):
+        sm = np.broadcast_to((labels == b)[None], (P["samp"][0],) + lines.shape).reshape(-1)
+        idx = np.flatnonzero(sm & (P["data"][0] != 0)).astype(np.int32)
+        bins.append((idx, np.ascontiguousarray(P["data"][:, idx])))
+    ctx = gpu_context(pkg, P)
+    ctx.set_num_bins(5)
+
+    def reload():
+        for b, (idx, yb) in enumerate(bins):
+            ctx.set_bin_samples(b, idx, yb)
+
+    print("autotune: tile", flush=True)
+    tcfg, tt = tune.tune_tile(ctx, reload)
+    assert ctx.tile_config()[0] == tcfg == min(tt, key=tt.get)
+    assert sorted(tt) == list(range(pkg.cgs.lib().cgs_num_tile_configs()))
+    print("autotune: group", flush=True)
+    g, tg = tune.tune_group(ctx)
+    assert 1 <= g <= 5 and g == min(tg, key=tg.get)
+    x, _, its = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=6)
+    assert ctx.bin_group() == g
+    for b, (idx, yb) in enumerate(bins):
+        d = np.zeros_like(P["data"])
+        d[:, idx] = yb
+        o.calculate_weights(d)
+        xr, _, itr = o.cg_sense(d, thresh=0.0, max_iter=6, prec=64)
+        assert its[b] == itr == 7
+        assert rel_l2(x[b], xr) < CG_TOL
+    o.calculate_weights()
+    o.exclude_missing_data(P["data"])
