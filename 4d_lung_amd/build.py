@@ -73,7 +73,7 @@ def _run(cmd):
 def build_reco():
     """The drop-in Reco / Fourier / IO / Trajectory classes (host C++ over the
     C-ABI) and the per-state driver program."""
-    flags = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall",
+    flags = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-pthread", "-ffp-contract=off", "-Wall",
              "-I" + INCLUDE, "-I" + RECO_INC]
     srcs = [os.path.join(RECO_SRC, f) for f in ("Reco.cpp", "Fourier.cpp", "IO.cpp", "Trajectory.cpp", "functions.cpp")]
     tmp = RECO_LIB + ".tmp"

@@ -18,8 +18,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <future>
 #include <iostream>
 #include <string>
+#include <utility>
 
 #include "IO.h"
 #include "Reco.h"
@@ -137,9 +139,24 @@ int main(int argc, char** argv) {
         reco.denCorr = IO::importDensityCorr(reco.getDensityCorrectionPath(), nSamplesPerCh);
     else if (reco.getDensityCorrection() == Reco::Density::VORONOI3D)
         reco.denCorr = estimateDensityV3D(reco.trajectory->kSpaceTraj, sx, sy, sz, false, 1.0);
+    // Buffered data loading as in the reference's main (CGSense.cpp:204-216):
+    // while a state is reconstructed, the next state's file is read and
+    // permuted into reco.dataBuffer on a host thread; the two buffers swap at
+    // the start of the next state.
+    const bool multipleData = nStates > 1;
+    if (multipleData) reco.allocData(&reco.dataBuffer);
     for (int n = 0; n < nStates; n++) {
+        std::future<bool> nextImport;   // its destructor waits for the read
+        if (multipleData) {
+            if (n != 0) std::swap(reco.data, reco.dataBuffer);
+            if (n < nStates - 1) {
+                fftwf_complex* buf = reco.dataBuffer;
+                const std::string path = reco.getBatchFileNum(n + 1);
+                nextImport = std::async(std::launch::async,
+                                        [&reco, buf, path]() { return IO::importDataBin(reco, buf, path); });
+            }
+        }
         if (n > 0) {
-            IO::importDataBin(reco, reco.data, reco.getBatchFileNum(n));
             reco.kernel->clearWeights();
             reco.kernel->calculateWeights();
         }

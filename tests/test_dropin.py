@@ -27,9 +27,11 @@ def write_case(d, P, thr=0.0, max_iter=9, dens=False, nstates=1, traj_kind=0):
         f.write("%d %d %d %d %d %d %d %.17g %d %d %d %d\n" % (tuple(grid) + tuple(samp) + (
             C, thr, max_iter, mode, nstates, traj_kind)))
     P["traj"].astype(np.float64).tofile(os.path.join(d, "traj.bin"))
-    # state file: coil slowest, then z line, y line, readout fastest (IO.cpp:85-93)
-    file_order = P["data"].reshape((C,) + tuple(samp)).transpose(0, 3, 2, 1)
-    np.ascontiguousarray(file_order).astype(np.complex64).tofile(os.path.join(d, "state0.bin"))
+    # state files: coil slowest, then z line, y line, readout fastest (IO.cpp:85-93);
+    # P["states"] (a list of C x nS arrays) gives one file per state, else state0.bin
+    for n, data in enumerate(P.get("states", [P["data"]])):
+        file_order = data.reshape((C,) + tuple(samp)).transpose(0, 3, 2, 1)
+        np.ascontiguousarray(file_order).astype(np.complex64).tofile(os.path.join(d, "state%d.bin" % n))
     for c in range(C):
         P["maps"][c].astype(np.complex64).tofile(os.path.join(d, "sensitivityMaps", "map%d.bin" % c))
     if dens is True:
@@ -171,3 +173,37 @@ def test_dropin_main_matches_oracle(orc, tmp_path, dens):
     rss = np.sqrt((np.abs(img) ** 2).sum(0))
     got = np.fromfile(os.path.join(d, "resultRssq.bin"), np.float32)
     assert rel_l2(got, rss) < 1e-5
+
+
+@pytest.mark.gpu
+def test_dropin_main_buffered_states_match_oracle(orc, tmp_path):
+    """Three respiratory states (disjoint line subsets of the acquisition) in
+    state0..2.bin: the driver reads the next state's file on a host thread
+    while the current one is reconstructed and swaps data / dataBuffer
+    (CGSense.cpp:204-216), recomputing the weights per state; every state's
+    image and pass count match the oracle's solve of that state's data."""
+    P = small_problem(orc, seed=23)
+    lines = P["lines"]
+    acq = np.flatnonzero(lines.ravel())
+    labels = np.full(lines.size, -1)
+    labels[acq] = np.arange(acq.size) % 3
+    labels = labels.reshape(lines.shape)
+    states = []
+    for s in range(3):
+        sm = np.broadcast_to((labels == s)[None], (P["samp"][0],) + lines.shape).reshape(-1)
+        states.append(np.where(sm[None, :], P["data"], 0).astype(np.complex64))
+    P["states"] = states
+    d = str(tmp_path)
+    write_case(d, P, max_iter=6, nstates=3)
+    out = run_driver(d)
+    assert out.count("End cgSense") == 3
+    its = open(os.path.join(d, "results", "performedIterations.txt")).read().split()
+    o = P["oracle"]
+    for s, data in enumerate(states):
+        o.calculate_weights(data)
+        xr, _, itr = o.cg_sense(data, thresh=0.0, max_iter=6, prec=64)
+        x = np.fromfile(os.path.join(d, "results", "b_state%d.bin" % s), np.complex64)
+        assert rel_l2(x, xr) < 1e-4, s
+        assert its[s] == str(itr)
+    o.calculate_weights()
+    o.exclude_missing_data(P["data"])
