@@ -83,6 +83,7 @@ def lib():
         "cgs_cg_prepare": (I, [P, D, I]),
         "cgs_cg_iterate": (I, [P, I, I]),
         "cgs_cg_results": (I, [P, P, P, P]),
+        "cgs_cg_history_length": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_cg_sense_bins": (I, [P, I, D, I, P, P, P]),
         "cgs_bin_count": (I, [P, I, ctypes.POINTER(LL)]),
         "cgs_bin_tables": (I, [P, I, P, P, P, P, P]),
@@ -137,6 +138,13 @@ def _c64(a, n=None):
     a = np.ascontiguousarray(a, dtype=np.complex64)
     if n is not None and a.size != n:
         raise ValueError("expected %d complex values, got %d" % (n, a.size))
+    return a
+
+
+def _f32(a, n, what):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.size != n:
+        raise ValueError("%s: expected %d values, got %d" % (what, n, a.size))
     return a
 
 
@@ -216,7 +224,7 @@ class Context:
         if dcf is None:
             self._ck(self._L.cgs_set_density(self.h, None))
         else:
-            d = np.ascontiguousarray(dcf, dtype=np.float32)
+            d = _f32(dcf, self.ns, "density weights")
             self._ck(self._L.cgs_set_density(self.h, _ptr(d)))
 
     def set_maps(self, maps):
@@ -285,25 +293,31 @@ class Context:
 
     def set_bin_data(self, b, data, dcf=None):
         d = _c64(data, self.ncoil * self.ns)
-        w = None if dcf is None else np.ascontiguousarray(dcf, dtype=np.float32)
+        w = None if dcf is None else _f32(dcf, self.ns, "density weights")
         self._ck(self._L.cgs_set_bin_data(self.h, int(b), _ptr(d), _ptr(w)))
 
     def set_bin_samples(self, b, idx, y, dcf=None):
         idx = np.ascontiguousarray(idx, dtype=np.int32)
         y = _c64(y, self.ncoil * idx.size)
-        w = None if dcf is None else np.ascontiguousarray(dcf, dtype=np.float32)
+        w = None if dcf is None else _f32(dcf, idx.size, "density weights")
         self._ck(self._L.cgs_set_bin_samples(self.h, int(b), idx.size, _ptr(idx), _ptr(y), _ptr(w)))
 
     def cg_prepare(self, thresh=0.0, max_iter=15):
-        self._max_iter = int(max_iter)
         self._ck(self._L.cgs_cg_prepare(self.h, float(thresh), int(max_iter)))
 
     def cg_iterate(self, mode, n_iter):
         self._ck(self._L.cgs_cg_iterate(self.h, int(mode), int(n_iter)))
 
+    def cg_history_length(self):
+        """Residuals per bin that cg_results returns (max_iter + 1 of the
+        last cg_prepare or solve)."""
+        n = ctypes.c_int()
+        self._ck(self._L.cgs_cg_history_length(self.h, ctypes.byref(n)))
+        return n.value
+
     def cg_results(self, want_x=True):
         x = np.zeros((self.nbins,) + self.grid, np.complex64) if want_x else None
-        res = np.zeros((self.nbins, self._max_iter + 1), np.float64)
+        res = np.zeros((self.nbins, self.cg_history_length()), np.float64)
         its = np.zeros(self.nbins, np.int32)
         self._ck(self._L.cgs_cg_results(self.h, _ptr(x), _ptr(res), _ptr(its)))
         return x, res, its

@@ -887,6 +887,10 @@ int cgs_set_img_range(cgs_ctx* h, const int range[3], const int offcenter[3]) {
         h->imgRange[a] = range[a];
         h->offc[a] = offcenter ? offcenter[a] : 0;
     }
+    // the crop box is baked into the right-hand sides and into the captured
+    // iteration: both are formed again by the next cgs_cg_prepare
+    drop_graph(h);
+    h->cg_ready = false;
     return CGS_OK;
 }
 
@@ -1252,6 +1256,9 @@ int cgs_cg_iterate(cgs_ctx* h, int mode, int n_iter) {
         if (h->prof) {
             for (int it = 0; it < n_iter; ++it) CKS(iterate_all_groups(h));
         } else {
+            if (h->gexec && (h->g_thresh != h->thresh || h->g_max_iter != h->max_iter || h->g_hist != h->d_hist ||
+                             h->g_group != h->group))
+                drop_graph(h);
             if (!h->gexec) {
                 CK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
                 const int st = iterate_all_groups(h);
@@ -1296,6 +1303,13 @@ int cgs_cg_results(cgs_ctx* h, float* x_out, double* resid_out, int* iters_out) 
     return CGS_OK;
 }
 
+int cgs_cg_history_length(const cgs_ctx* h, int* n) {
+    if (!h || !n) return CGS_ERR_ARG;
+    if (!h->cg_ready) return CGS_ERR_STATE;
+    *n = h->hist_stride;
+    return CGS_OK;
+}
+
 int cgs_cg_sense_bins(cgs_ctx* h, int mode, double thresh, int max_iter, float* x_out, double* resid_out,
                       int* iters_out) {
     CKS(cgs_cg_prepare(h, thresh, max_iter));
@@ -1314,6 +1328,9 @@ int cgs_cg_sense_images(cgs_ctx* h, const float* img, double thresh, int max_ite
     h->group = 1;
     h->thresh = thresh;
     h->max_iter = max_iter;
+    // a captured batched iteration refers to the loop control and history
+    // buffer of the last cgs_cg_prepare: this plain solve replaces both
+    drop_graph(h);
     if (!h->d_hist || h->hist_stride != max_iter + 1 || h->hist_bins != 1) {
         dfree(h, h->d_hist, 1);
         h->hist_stride = max_iter + 1;

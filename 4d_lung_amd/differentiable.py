@@ -55,7 +55,6 @@ class Encoding:
         self.scale = 1e-3 / math.sqrt(self.ng)
         self._stream = None
         self._coords = None
-        self._coords_version = None
         self.m = 0
 
     def close(self):
@@ -83,18 +82,23 @@ class Encoding:
             self._stream = s
 
     def set_coords(self, k):
-        """Bin the ``(m, 3)`` float32 coordinates (kept until another tensor,
-        or a new version of this one, is given)."""
-        if k is self._coords and k._version == self._coords_version:
-            return
+        """Bin the ``(m, 3)`` float32 coordinates (kept until coordinates of
+        other values are given)."""
         if k.dtype != torch.float32 or not k.is_cuda or k.dim() != 2 or k.shape[1] != 3:
             raise ValueError("coordinates must be a (m, 3) float32 CUDA tensor")
-        k = k.detach().contiguous()
+        kd = k.detach()
+        # The binning (sort, KB tables, a stream synchronisation) is redone
+        # only when the coordinates change.  A private copy of the last binned
+        # coordinates is compared with the caller's tensor by value, so the
+        # cache also holds for the detached copies that autograd and
+        # checkpointing hand back, and an in-place change is never missed.
         self._stream_sync()
-        self._ck(self._L.cgs_pts_set_coords(self.p, k.shape[0], ctypes.c_void_p(k.data_ptr())))
-        self.m = int(k.shape[0])
-        self._coords = k
-        self._coords_version = k._version
+        if self._coords is not None and self._coords.shape == kd.shape and torch.equal(kd, self._coords):
+            return
+        kc = kd.contiguous()
+        self._ck(self._L.cgs_pts_set_coords(self.p, kc.shape[0], ctypes.c_void_p(kc.data_ptr())))
+        self.m = int(kc.shape[0])
+        self._coords = kc.clone() if kc.data_ptr() == k.data_ptr() else kc
 
     @staticmethod
     def _cplx(t, n):

@@ -36,9 +36,11 @@ CONFIGS = {
                                R=(3, 2), shift=1, cycles=6, amp=4.0, iters=20, bins=1, seed=1002),
     # Batched 4D respiratory-resolved: 128x128x96 image, 16 coils, R=2x2, 8 bins
     # holding 6-20 % of lines each, 15 CG iterations.
+    # Every bin is checked against the FP64 oracle at the workload's own 15
+    # iterations (about 20 s of oracle time per bin on 16 host cores).
     "4d_128x128x96": dict(image=(128, 128, 96), grid=(256, 256, 192), samp=(256, 128, 96), coils=16,
                           R=(2, 2), shift=1, cycles=3, amp=4.0, iters=15, bins=8, bin_frac=(0.06, 0.20),
-                          seed=1003, dcf="voronoi"),
+                          seed=1003, dcf="voronoi", check_iters=15, check_bins="all"),
     # Differentiable unrolled recon: 96x96x64 image, 8 coils, R=2x2, 5 unrolled iterations.
     "unrolled_96x96x64": dict(image=(96, 96, 64), grid=(192, 192, 128), samp=(192, 96, 64), coils=8,
                               R=(2, 2), shift=1, cycles=3, amp=4.0, iters=5, bins=1, seed=1004),
@@ -49,7 +51,7 @@ CONFIGS = {
     # No density weights (the reference's default, Density::NONE).
     "4d_256x256x160": dict(image=(256, 256, 160), grid=(512, 512, 320), samp=(512, 256, 160), coils=32,
                            R=(3, 3), shift=1, cycles=6, amp=4.0, iters=30, bins=20, bin_frac=(0.04, 0.09),
-                           seed=1005, check_iters=4, dcf="none"),
+                           seed=1005, check_iters=4, check_bins="0", dcf="none"),
 }
 
 
@@ -227,3 +229,62 @@ def add_noise(y, sample_mask, sigma_rel, seed):
     z = m & (y[0] == 0)
     y[0, z] = np.complex64(sig)
     return y
+
+
+def binned_workload(pkg, cfg, log=None, pruning=-1):
+    """Inputs of a respiratory-resolved workload on the device.
+
+    The phantom's k-space is formed with the library's own forward operator E
+    on every acquired sample (a context ``ctx`` of the workload's grid,
+    samples and coils with trajectory, kernelFT and maps set), noise is added,
+    and the acquired lines are split into the seeded respiratory bins; every
+    bin keeps its samples in compact form (original indices, ``C x m``
+    values) and, for ``dcf="voronoi"``, the Voronoi weights of its own
+    samples (cgs_voronoi_density on the device).  Returns a dict with ``ctx``,
+    ``traj``, ``maps``, ``labels`` and ``bins`` = [(idx, values, weights)] and
+    ``voronoi_seconds`` (None without density weights).
+    """
+    import time
+
+    say = log or (lambda _m: None)
+    grid, samp, image, C, B = cfg["grid"], cfg["samp"], cfg["image"], cfg["coils"], cfg["bins"]
+    ns = int(np.prod(samp))
+    t0 = time.time()
+    traj = wave_caipi_trajectory(grid, samp, cfg["cycles"], cfg["amp"])
+    lines = caipi_mask(samp[1], samp[2], cfg["R"], cfg["shift"])
+    labels = respiratory_bins(lines, B, cfg["bin_frac"], cfg["seed"] + 5)
+    smask = sample_mask_from_lines(lines, samp[0])
+    maps = coil_maps(grid, C, cfg["seed"] + 1)
+    rho = phantom(grid, image, cfg["seed"] + 2).reshape(-1)
+    say("synthetic image and maps %.1fs (%d lines in %d bins)" % (time.time() - t0, int(lines.sum()), B))
+    ctx = pkg.Context(grid, samp, C)
+    if pruning >= 0:
+        ctx.set_pruning(pruning)
+    ctx.set_trajectory(traj)
+    ctx.calculate_kernel_ft()
+    ctx.set_maps(maps)
+    mask_data = np.zeros((1, ns), np.complex64)
+    mask_data[0, smask] = 1
+    ctx.exclude_missing_data(mask_data)
+    del mask_data
+    y = ctx.nufft_fwd(maps * rho[None, :])
+    data = add_noise(y, smask, 1e-3, cfg["seed"] + 3)
+    del y
+    say("k-space synthesized %.1fs" % (time.time() - t0))
+    bins = []
+    tv = time.time()
+    voronoi = cfg.get("dcf", "voronoi") == "voronoi"
+    for b in range(B):
+        idx = np.flatnonzero(sample_mask_from_lines(labels == b, samp[0])).astype(np.int32)
+        w = pkg.cgs.voronoi_density(traj[idx]) if voronoi else None
+        bins.append((idx, np.ascontiguousarray(data[:, idx]), w))
+        if voronoi:
+            say("bin %d: %d samples, Voronoi weights after %.1fs" % (b, idx.size, time.time() - tv))
+    counts = [int(bb[0].size) for bb in bins]
+    voronoi_s = time.time() - tv if voronoi else None
+    if voronoi:
+        say("per-bin Voronoi density weights: %.1fs for %d samples (bins of %d..%d)"
+            % (voronoi_s, sum(counts), min(counts), max(counts)))
+    else:
+        say("bins of %d..%d samples, no density weights" % (min(counts), max(counts)))
+    return dict(ctx=ctx, traj=traj, maps=maps, labels=labels, bins=bins, voronoi_seconds=voronoi_s)

@@ -1,20 +1,27 @@
 #!/usr/bin/env python
-"""CG-SENSE iterations/s on one MI355X.
+"""CG-SENSE bin-iterations/s on one MI355X.
 
-Workload (default): the single-frame full-size case -- 160x160x112 image on a
-320x320x224 grid, 16 coils, R=3x2 wave-CAIPI (2987 of 17920 lines x 320
-samples, 6 wave cycles), k-space synthesized from a seeded phantom x smooth
-coil maps + complex noise, FP32 gridding/FFT with FP64 CG scalars.  A step is
-one CG-SENSE iteration: one E^H W E application over all coils plus the CG
-update (Reco::cgSenseDynamic loop body, Reco.cpp:230-311).  Inputs are
-resident in HBM before the timed region; the right-hand side E^H W y is formed
-before it.
+Workload (default): the batched 4D respiratory-resolved case -- 128x128x96
+image on a 256x256x192 grid, 16 coils, R=2x2 wave-CAIPI lines (3072 of 12288,
+256 samples per line, 3 wave cycles) assigned to 8 respiratory bins by a
+seeded respiratory trace (6-20 % of the lines per bin), per-bin Voronoi
+density weights, k-space synthesized from a seeded phantom x smooth coil maps
++ complex noise, FP32 gridding/FFT with FP64 CG scalars, 15 CG iterations.
+Every bin is its own CG-SENSE solve (the reference's per-state loop,
+CGSense.cpp:200-282).  A step is one CG-SENSE iteration of every bin: one
+E^H W E application over all coils plus the CG update (Reco::cgSenseDynamic
+loop body, Reco.cpp:230-311), counted once per bin.  Inputs are resident in
+HBM before the timed region; the right-hand sides E^H W y are formed before
+it.  The plain path (bins one after another, host reads the residual every
+iteration, like the reference's loop) is timed beside it.
 
-The JSON line carries the measured rate, the relative L2 error of the final
-image against the FP64 CPU oracle after the same iterations, the roofline of
-the dominant kernel (HIP-event timing on the library's stream, algorithmic
-bytes per launch) and the CPU baseline (the oracle's FP32 restatement of the
-reference loops, timed on this host for one iteration).
+The JSON line carries the measured rate, the relative L2 error of every bin's
+image against the FP64 CPU oracle after the workload's iteration count, the
+roofline of the dominant kernel (HIP-event timing on the library's stream,
+algorithmic bytes per launch) and the CPU baseline (the oracle's FP32
+statement of the reference loops timed on this host).  --workload selects
+the other configurations (single_160x160x112, plain_64x64x48,
+4d_256x256x160).
 """
 import argparse
 import importlib
@@ -75,6 +82,22 @@ def touched_cells(k, grid, kw):
     return int(mark.sum())
 
 
+def survey_model(cfg, counts, rate):
+    """The per-iteration byte model that counts every 3D FFT once (one read
+    and one write of the coil grids): per bin and iteration
+    C*(56*Ng + 48*Ni + 44*M_b) + 72*Ni bytes (pad/expand 8 Ng, FFT 16 Ng,
+    degrid 8 Ng + 24 M_b, grid 20 M_b + 8 Ng, inverse FFT 16 Ng, crop / conj /
+    coil sum 32 Ni per coil, CG vector work 72 Ni).  rate: bin-iterations/s."""
+    C = cfg["coils"]
+    ng = int(np.prod(cfg["grid"]))
+    ni = int(np.prod(cfg["image"]))
+    per_bin = [C * (56 * ng + 48 * ni + 44 * m) + 72 * ni for m in counts]
+    avg = float(np.mean(per_bin))
+    gbs = avg * rate / 1e9
+    return {"bytes_per_batched_iteration": float(sum(per_bin)), "gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+            "ceiling_bin_iterations_per_s": HBM_PEAK_GBS * 1e9 / avg}
+
+
 def stage_rates(rep, n_iter, bytes_per_iter):
     """Per-stage launch count, average launch ms and algorithmic GB/s from a
     profile report of n_iter iterations (bytes_per_iter: stage -> bytes one
@@ -120,48 +143,10 @@ def bench_binned(args, cfg, pkg, torch):
     grid, samp, image, C, B = cfg["grid"], cfg["samp"], cfg["image"], cfg["coils"], cfg["bins"]
     ng, ns = int(np.prod(grid)), int(np.prod(samp))
     K, W = args.steps, args.warmup
-    t0 = time.time()
-    traj = synth.wave_caipi_trajectory(grid, samp, cfg["cycles"], cfg["amp"])
-    lines = synth.caipi_mask(samp[1], samp[2], cfg["R"], cfg["shift"])
-    labels = synth.respiratory_bins(lines, B, cfg["bin_frac"], cfg["seed"] + 5)
-    smask = synth.sample_mask_from_lines(lines, samp[0])
-    maps = synth.coil_maps(grid, C, cfg["seed"] + 1)
-    rho = synth.phantom(grid, image, cfg["seed"] + 2).reshape(-1)
-    log("synthetic image and maps %.1fs (%d lines in %d bins)" % (time.time() - t0, int(lines.sum()), B))
-    ctx = pkg.Context(grid, samp, C)
-    if args.pruning >= 0:
-        ctx.set_pruning(args.pruning)
-    ctx.set_trajectory(traj)
-    ctx.calculate_kernel_ft()
-    ctx.set_maps(maps)
-    mask_data = np.zeros((1, ns), np.complex64)
-    mask_data[0, smask] = 1
-    ctx.exclude_missing_data(mask_data)
-    del mask_data
-    img = maps * rho[None, :]
-    y = ctx.nufft_fwd(img)
-    del img
-    data = synth.add_noise(y, smask, 1e-3, cfg["seed"] + 3)
-    del y
-    log("k-space synthesized %.1fs" % (time.time() - t0))
-    # per-bin compact samples and (per the workload) Voronoi weights
-    bins = []
-    tv = time.time()
-    voronoi = cfg.get("dcf", "voronoi") == "voronoi"
-    for b in range(B):
-        idx = np.flatnonzero(synth.sample_mask_from_lines(labels == b, samp[0])).astype(np.int32)
-        w = pkg.cgs.voronoi_density(traj[idx]) if voronoi else None
-        bins.append((idx, np.ascontiguousarray(data[:, idx]), w))
-        if voronoi:
-            log("bin %d: %d samples, Voronoi weights after %.1fs" % (b, idx.size, time.time() - tv))
+    wl = synth.binned_workload(pkg, cfg, log=log, pruning=args.pruning)
+    ctx, traj, maps, bins, voronoi_s = wl["ctx"], wl["traj"], wl["maps"], wl["bins"], wl["voronoi_seconds"]
+    voronoi = voronoi_s is not None
     counts = [int(bb[0].size) for bb in bins]
-    voronoi_s = time.time() - tv if voronoi else None
-    if voronoi:
-        log("per-bin Voronoi density weights: %.1fs for %d samples (bins of %d..%d)"
-            % (voronoi_s, sum(counts), min(counts), max(counts)))
-    else:
-        log("bins of %d..%d samples, no density weights" % (min(counts), max(counts)))
-    del data
     ctx.set_num_bins(B)
 
     def load_bins():
@@ -248,15 +233,19 @@ def bench_binned(args, cfg, pkg, torch):
         "batched_over_plain": value / plain_value if plain_value else None,
         "roofline": roofline_of(kernels, args.workload, B),
         "algorithmic_gbs_per_iteration": total_bytes * value / B / 1e9,
+        "survey_model": survey_model(cfg, counts, value),
         "voronoi_seconds": voronoi_s,
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "launches_per_iteration": v["count"] // n_prof,
                         "gbs": round(v["gbs"], 1) if v["gbs"] else None} for k, v in kernels.items()},
     }
     ci = K if args.check_iters is None else args.check_iters
-    check_bins = list(range(B)) if args.check_all_bins else [int(v) for v in args.check_bins.split(",") if v]
+    if args.check_all_bins or args.check_bins == "all":
+        check_bins = list(range(B))
+    else:
+        check_bins = [int(v) for v in args.check_bins.split(",") if v]
     if not args.no_check and ci != K:
-        # the oracle's FP64 solve of the full iteration count does not fit the
-        # run: both sides solve the checked bins for ci iterations instead
+        # the FP64 check runs the workload's own iteration count (or what the
+        # oracle's time allows): the GPU solves every bin again for that many
         ctx.cg_prepare(0.0, ci - 1)
         ctx.cg_iterate(pkg.SOLVE_BATCHED, ci)
         x, _, its = ctx.cg_results()
@@ -283,44 +272,60 @@ def bench_binned(args, cfg, pkg, torch):
         o.set_density(wf)
         return o, d
 
+    ref0 = None
     if not args.no_check:
         errs = {}
         for b in check_bins:
             tc = time.time()
             o, d = oracle_for(b)
-            xr, _, _ = o.cg_sense(d, 0.0, ci - 1, prec=64)
+            xr, _, itr = o.cg_sense(d, 0.0, ci - 1, prec=64)
+            assert itr == ci, (b, itr)
             errs[b] = float(np.linalg.norm(x[b].ravel() - xr.ravel()) / np.linalg.norm(xr.ravel()))
             log("FP64 oracle check bin %d after %d iterations: rel-L2 %.3e (%.0fs)" % (b, ci, errs[b], time.time() - tc))
+            if b == 0:
+                ref0 = xr
             del o, d, xr
         out["rel_l2_vs_fp64_oracle_per_bin"] = errs
+        out["rel_l2_vs_fp64_oracle_max"] = max(errs.values())
         out["check_iterations"] = ci
+        out["checked_bins"] = len(errs)
         out["within_tolerance"] = bool(max(errs.values()) <= 1e-4)
     del x
     if not args.no_cpu:
-        # bin 0 on the CPU: timed CG iterations of the FP32 restatement (the
-        # loop timer covers the iteration, not the right-hand side); median of
-        # the timed runs after one warm-up run
-        o, d = oracle_for(0)
-        times = []
-        reps = 6
-        for r in range(reps):
-            _, _, itc, loop_s = o.cg_sense(d, 0.0, 0, prec=32, timed=True)
-            if r == 0 and loop_s / itc > 5.0:
-                reps = 3
-            if r > 0:
-                times.append(loop_s / itc)
-            if r + 1 >= reps:
-                break
-        med = float(np.median(times))
-        out["cpu_baseline"] = {"value": 1.0 / med, "unit": "bin-iterations/s", "cores": threads, "kind": "port",
-                               "sample": "median of %d timed CG iterations (after 1 warm-up) of bin 0 of the same "
-                                         "workload, FP32 restatement of Reco.cpp:175-553 (coil-parallel OpenMP, own "
-                                         "CPU FFT; FFTW is not available, so this understates the reference's FFT "
-                                         "speed)" % len(times)}
-        log("CPU baseline: %.4f bin-it/s on %d threads (median of %s s)" % (
-            1.0 / med, threads, ", ".join("%.2f" % t for t in times)))
-        del o, d
+        out["cpu_baseline"] = cpu_baseline(orc, oracle_for, ci, ref0, threads, "bin 0 of the same workload",
+                                           "bin-iterations/s")
     return out
+
+
+def cpu_baseline(orc, oracle_for, ci, ref0, threads, what, unit, budget_s=30.0):
+    """The reference's CPU loop (the oracle's FP32 statement of
+    Reco.cpp:175-553: per-sample tap lists, coil-parallel OpenMP, float
+    atomic coil sum as Reco.cpp:120-129, own CPU FFT) on one dataset of the
+    workload: a warm-up solve of one iteration, then a timed solve of the
+    check's iteration count when it fits the time budget (else of as many
+    iterations as fit); the median of the per-iteration times.  With the
+    full iteration count its image's relative L2 error against the FP64
+    oracle is reported too."""
+    o, d = oracle_for(0)
+    o.set_atomic_coil_sum(True)
+    _, _, _, warm_s, warm_it = o.cg_sense(d, 0.0, 0, prec=32, timed=True, iter_times=True)
+    per_it = float(warm_it[0])
+    n = ci if per_it * ci <= budget_s else max(2, int(budget_s / per_it))
+    x32, _, itc, loop_s, per = o.cg_sense(d, 0.0, n - 1, prec=32, timed=True, iter_times=True)
+    med = float(np.median(per))
+    rec = {"value": 1.0 / med, "unit": unit, "cores": threads, "kind": "port",
+           "sample": "median of %d timed CG iterations (after a 1-iteration warm-up solve) of %s: FP32 "
+                     "statement of the reference loop Reco.cpp:175-553 (per-sample tap lists, coil-parallel "
+                     "OpenMP, float omp-atomic coil sum as Reco.cpp:120-129, own CPU FFT; FFTW is not available, "
+                     "so this understates the reference's FFT speed)" % (len(per), what),
+           "iterations": int(itc), "seconds": float(loop_s)}
+    if ref0 is not None and n == ci:
+        rec["rel_l2_vs_fp64_oracle"] = float(np.linalg.norm(x32.ravel() - ref0.ravel()) / np.linalg.norm(ref0.ravel()))
+    log("CPU baseline: %.4f %s on %d threads (median of %d iterations, %.1f s)%s" % (
+        rec["value"], unit, threads, len(per), loop_s,
+        ", rel-L2 vs FP64 %.2e" % rec["rel_l2_vs_fp64_oracle"] if "rel_l2_vs_fp64_oracle" in rec else ""))
+    del o, d
+    return rec
 
 
 def main():
@@ -328,12 +333,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed CG iterations (default: the workload's own)")
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="single_160x160x112")
+    ap.add_argument("--workload", default="4d_128x128x96")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 oracle accuracy check")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--tile", type=int, default=-1, help="gridding tile config (-1: autotune)")
     ap.add_argument("--check-all-bins", action="store_true", help="binned workloads: oracle check of every bin")
-    ap.add_argument("--check-bins", default="0", help="binned workloads: bins checked against the oracle")
+    ap.add_argument("--check-bins", default=None,
+                    help="binned workloads: bins checked against the oracle, comma separated or 'all' "
+                         "(default: the workload's own setting)")
     ap.add_argument("--check-iters", type=int, default=None,
                     help="binned workloads: iterations of the oracle check (default: the workload's own, else "
                          "--steps); with fewer than --steps the checked bins are solved again for that many")
@@ -368,6 +375,8 @@ def main():
         args.steps = cfg["iters"]
     if args.check_iters is None:
         args.check_iters = cfg.get("check_iters")
+    if args.check_bins is None:
+        args.check_bins = cfg.get("check_bins", "0")
     if cfg["bins"] != 1:
         print(json.dumps(bench_binned(args, cfg, pkg, torch)), flush=True)
         return
@@ -503,6 +512,7 @@ def main():
         # every stage runs once per iteration: the iteration's algorithmic
         # bytes times the iteration rate
         "algorithmic_gbs_per_iteration": sum(v["bytes"] or 0 for v in kernels.values()) * value / 1e9,
+        "survey_model": survey_model(cfg, [m], value),
         "kernels": {k: {"avg_ms": round(v["avg_ms"], 4), "gbs": round(v["gbs"], 1) if v["gbs"] else None}
                     for k, v in kernels.items()},
     }
@@ -524,27 +534,20 @@ def main():
         out["rel_l2_vs_fp64_oracle"] = err
         out["within_tolerance"] = bool(err <= 1e-4)
         log("FP64 oracle check: rel-L2 %.3e (%.0fs)" % (err, time.time() - tc))
+        ref0 = xr
         del o
-    if not args.no_cpu:
-        # median of 5 single-iteration CG runs after one warm-up run (the loop
-        # timer covers the iteration only, not the RHS formation)
+
+    def oracle_for(_b):
         o = orc.OracleReco(grid, samp, C, nthreads=threads)
         o.set_trajectory(traj)
         o.calculate_weights(data)
         o.calculate_kernel_ft()
         o.set_maps(maps)
-        times = []
-        for rep in range(6):
-            _, _, itc, loop_s = o.cg_sense(data, 0.0, 0, prec=32, timed=True)
-            if rep > 0:
-                times.append(loop_s / itc)
-        med = float(np.median(times))
-        out["cpu_baseline"] = {"value": 1.0 / med, "unit": "iterations/s", "cores": threads, "kind": "port",
-                               "sample": "median of 5 timed CG iterations (after 1 warm-up) of the same workload, "
-                                         "FP32 restatement of Reco.cpp:175-553 (coil-parallel OpenMP, own CPU FFT; "
-                                         "FFTW is not available, so this understates the reference's FFT speed)"}
-        log("CPU baseline: %.3f it/s on %d threads (median of %s s)" % (1.0 / med, threads,
-                                                                        ", ".join("%.2f" % t for t in times)))
+        return o, data
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(orc, oracle_for, K, None if args.no_check else ref0, threads,
+                                           "the same workload", "iterations/s")
     print(json.dumps(out), flush=True)
 
 

@@ -129,9 +129,13 @@ int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter);
  * until every bin stopped or n_iter iterations were issued; returns after the
  * device finished. */
 int cgs_cg_iterate(cgs_ctx* h, int mode, int n_iter);
-/* Results: x_out nbins*Ng complex (may be NULL), resid_out nbins*(max_iter+1),
- * iters_out nbins. */
+/* Results: x_out nbins*Ng complex (may be NULL), resid_out nbins*n doubles
+ * with n from cgs_cg_history_length (max_iter + 1 of the last prepare or
+ * solve), iters_out nbins. */
 int cgs_cg_results(cgs_ctx* h, float* x_out, double* resid_out, int* iters_out);
+/* Residual history length per bin (Reco::cgResiduals, Reco.cpp:299) of the
+ * current CG state: the doubles per bin that cgs_cg_results writes. */
+int cgs_cg_history_length(const cgs_ctx* h, int* n);
 /* prepare + iterate(max_iter + 1) + results. */
 int cgs_cg_sense_bins(cgs_ctx* h, int mode, double thresh, int max_iter, float* x_out, double* resid_out,
                       int* iters_out);

@@ -299,6 +299,10 @@ struct Oracle {
     std::vector<float> kft;                        // ConvKernel::kernelFT real part (Reco.h:60)
     std::vector<float> dcf;                        // Reco::denCorr (Reco.h:86)
     std::vector<Cplx<float>> maps;                 // Reco::sMaps (Reco.h:84), C x Ng
+    // Coil sum in the reference's own loop shape (float omp atomic adds of
+    // each coil into q, in whatever order the threads reach them) instead of
+    // the fixed coil order; used to time the reference's CPU loop.
+    bool atomic_sum = false;
 };
 
 // Reco::ConvKernel::collectNeighborIndices (Reco.cpp:789-806)
@@ -618,6 +622,7 @@ void orc_exclude_missing_data(void* h, const float* data) {
 
 // Smooth-kernel mode for finite-difference checks (set before the weights).
 void orc_set_exact_weights(void* h, int on) { ((Oracle*)h)->exact_w = on != 0; }
+void orc_set_atomic_coil_sum(void* h, int on) { ((Oracle*)h)->atomic_sum = on != 0; }
 
 long orc_num_taps(void* h) {
     Oracle* o = (Oracle*)h;
@@ -740,9 +745,32 @@ int apply_normal(Oracle& o, const Cplx<T>* p, Cplx<T>* q, std::vector<Cplx<T>>& 
 #pragma omp parallel for num_threads(nt) schedule(dynamic)
     for (int c = 0; c < o.C; c++) nufft_bwd_coil(o, data.data() + (size_t)c * nS, img.data() + (size_t)c * Ng);
     // mutliplyByConjSenseMap + addToSumOfImages (Reco.cpp:107-129, 257-262).
-    // The reference sums coils with float atomics in arbitrary order; here the
-    // coil order is fixed (0..C-1) and the grid is split across threads.
+    // The reference sums coils with float atomics in arbitrary order; by
+    // default the coil order is fixed here (0..C-1) and the grid is split
+    // across threads.
     for (long i = 0; i < Ng; i++) { q[i].re = 0; q[i].im = 0; }
+    if (o.atomic_sum) {
+        // the reference's loop: one coil per thread, conj-map multiply in
+        // place, then element-wise atomic adds into q (Reco.cpp:257-262)
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int c = 0; c < o.C; c++) {
+            const Cplx<float>* s = o.maps.data() + (size_t)c * Ng;
+            Cplx<T>* ic = img.data() + (size_t)c * Ng;
+            for (long i = 0; i < Ng; i++) {
+                const double re = ic[i].re, im = ic[i].im;
+                ic[i].re = (T)(re * s[i].re + im * s[i].im);
+                ic[i].im = (T)(-re * s[i].im + im * s[i].re);
+            }
+            for (long i = 0; i < Ng; i++) {
+#pragma omp atomic
+                q[i].re += ic[i].re;
+#pragma omp atomic
+                q[i].im += ic[i].im;
+            }
+        }
+        if (!std::isfinite((double)q[0].re) || !std::isfinite((double)q[0].im)) return 3;  // Reco.cpp:264-268
+        return 0;
+    }
 #pragma omp parallel for num_threads(nt) schedule(static)
     for (long i = 0; i < Ng; i++) {
         for (int c = 0; c < o.C; c++) {
@@ -762,7 +790,7 @@ int apply_normal(Oracle& o, const Cplx<T>* p, Cplx<T>* q, std::vector<Cplx<T>>& 
 // Reco::cgSenseDynamic (Reco.cpp:175-321) from the coil images img_c = E^H W y.
 template <typename T>
 int cg_sense(Oracle& o, std::vector<Cplx<T>>& img, double thresh, int max_iter, Cplx<T>* x,
-             double* resid_out, int* iters_out, double* loop_seconds = nullptr) {
+             double* resid_out, int* iters_out, double* loop_seconds = nullptr, double* iter_seconds = nullptr) {
     const long Ng = o.Ng;
     const int nt = o.nthreads;
     std::vector<Cplx<T>> a(Ng), q(Ng), p(Ng), r(Ng);
@@ -782,6 +810,7 @@ int cg_sense(Oracle& o, std::vector<Cplx<T>>& img, double thresh, int max_iter, 
     int iteration = 0;
     double current = 1000;
     const double t0 = omp_get_wtime();
+    double ti = t0;
     while (current > thresh) {
         const int st = apply_normal(o, p.data(), q.data(), img, data);
         if (st) { *iters_out = iteration; return st; }
@@ -817,6 +846,11 @@ int cg_sense(Oracle& o, std::vector<Cplx<T>>& img, double thresh, int max_iter, 
             p[i].re = (T)((double)r[i].re + be[0] * pr - be[1] * pi);
             p[i].im = (T)((double)r[i].im + be[0] * pi + be[1] * pr);
         }
+        if (iter_seconds) {
+            const double tn = omp_get_wtime();
+            iter_seconds[iteration] = tn - ti;
+            ti = tn;
+        }
         iteration++;
         if (iteration > max_iter) break;
     }
@@ -847,7 +881,7 @@ int orc_apply_normal(void* h, int prec, const void* p, void* q) {
 // resid_out: at least max_iter+1 doubles.  max_iter is the reference's
 // nMaxIter (Reco.cpp:228): at most max_iter+1 passes are made.
 int orc_cg_sense_timed(void* h, int prec, const float* data, double thresh, int max_iter, void* x_out,
-                       double* resid_out, int* iters_out, double* loop_seconds) {
+                       double* resid_out, int* iters_out, double* loop_seconds, double* iter_seconds) {
     Oracle* o = (Oracle*)h;
     if (o->kft.empty() || o->maps.empty()) return 2;
     const size_t nd = (size_t)o->C * o->nS;
@@ -857,16 +891,18 @@ int orc_cg_sense_timed(void* h, int prec, const float* data, double thresh, int 
         std::vector<Cplx<double>> d(nd), img((size_t)o->C * o->Ng);
         for (size_t i = 0; i < nd; i++) { d[i].re = dc[2 * i]; d[i].im = dc[2 * i + 1]; }
         orc_nufft_bwd(h, 64, d.data(), img.data());
-        return cg_sense(*o, img, thresh, max_iter, (Cplx<double>*)x_out, resid_out, iters_out, loop_seconds);
+        return cg_sense(*o, img, thresh, max_iter, (Cplx<double>*)x_out, resid_out, iters_out, loop_seconds,
+                        iter_seconds);
     }
     std::vector<Cplx<float>> img((size_t)o->C * o->Ng);
     orc_nufft_bwd(h, 32, dc.data(), img.data());
-    return cg_sense(*o, img, thresh, max_iter, (Cplx<float>*)x_out, resid_out, iters_out, loop_seconds);
+    return cg_sense(*o, img, thresh, max_iter, (Cplx<float>*)x_out, resid_out, iters_out, loop_seconds,
+                    iter_seconds);
 }
 
 int orc_cg_sense(void* h, int prec, const float* data, double thresh, int max_iter, void* x_out,
                  double* resid_out, int* iters_out) {
-    return orc_cg_sense_timed(h, prec, data, thresh, max_iter, x_out, resid_out, iters_out, nullptr);
+    return orc_cg_sense_timed(h, prec, data, thresh, max_iter, x_out, resid_out, iters_out, nullptr, nullptr);
 }
 
 // std::cyl_bessel_i(0, x) of libstdc++ (C++17 special functions), a second

@@ -67,7 +67,8 @@ def lib():
         L.orc_cg_sense.argtypes = [P, ctypes.c_int, P, ctypes.c_double, ctypes.c_int, P, P, P]
         L.orc_cg_sense_timed.restype = ctypes.c_int
         L.orc_cg_sense_timed.argtypes = [P, ctypes.c_int, P, ctypes.c_double, ctypes.c_int, P, P, P,
-                                         ctypes.POINTER(ctypes.c_double)]
+                                         ctypes.POINTER(ctypes.c_double), P]
+        L.orc_set_atomic_coil_sum.argtypes = [P, ctypes.c_int]
         L.orc_set_verbose.argtypes = [ctypes.c_int]
         L.orc_i0_std.restype = ctypes.c_double
         L.orc_i0_std.argtypes = [ctypes.c_double]
@@ -255,18 +256,30 @@ class OracleReco:
             raise RuntimeError("oracle apply_normal status %d" % st)
         return q
 
-    def cg_sense(self, data, thresh=0.0, max_iter=15, prec=64, timed=False):
+    def set_atomic_coil_sum(self, on=True):
+        """Coil sum in the reference's loop shape: float omp atomic adds of
+        every coil into q (Reco.cpp:120-129, 257-262), order left to the
+        threads; the default sums the coils in a fixed order."""
+        lib().orc_set_atomic_coil_sum(self.h, int(bool(on)))
+
+    def cg_sense(self, data, thresh=0.0, max_iter=15, prec=64, timed=False, iter_times=False):
         """Reco::cgSenseDynamic after density correction and nufftBWD of data.
-        With timed=True also returns the seconds spent in the iteration loop."""
+        With timed=True also returns the seconds spent in the iteration loop;
+        with iter_times=True, after that, the seconds of every iteration."""
         dt = np.complex128 if prec == 64 else np.complex64
         d = np.ascontiguousarray(data, dtype=np.complex64).reshape(self.ncoil, self.ns)
         x = np.zeros(self.ng, dt)
         res = np.zeros(max_iter + 2, np.float64)
+        per = np.zeros(max_iter + 2, np.float64)
         it = ctypes.c_int(0)
         secs = ctypes.c_double(0.0)
         st = lib().orc_cg_sense_timed(self.h, prec, _ptr(d), float(thresh), int(max_iter), _ptr(x), _ptr(res),
-                                      ctypes.byref(it), ctypes.byref(secs))
+                                      ctypes.byref(it), ctypes.byref(secs), _ptr(per))
         if st:
             raise RuntimeError("oracle cg_sense status %d" % st)
         out = (x.reshape(self.grid), res[: it.value], it.value)
-        return out + (secs.value,) if timed else out
+        if timed:
+            out = out + (secs.value,)
+        if iter_times:
+            out = out + (per[: it.value],)
+        return out

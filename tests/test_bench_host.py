@@ -123,3 +123,19 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=300)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_survey_model_totals():
+    """The per-iteration byte model that counts each 3D FFT once gives the
+    totals quoted for the workloads: 23.63 GB per single-frame iteration,
+    101.3 GB per batched iteration of the 8-bin 4D workload (whatever the
+    split of its 786,432 samples per coil over the bins)."""
+    synth = importlib.import_module("4d_lung_amd.synth")
+    c2 = bench.survey_model(synth.CONFIGS["single_160x160x112"], [955733], 1.0)
+    assert abs(c2["bytes_per_batched_iteration"] / 1e9 - 23.63) < 0.005
+    parts = [50000, 120000, 90000, 100000, 146432, 80000, 100000, 100000]
+    assert sum(parts) == 786432
+    c3 = bench.survey_model(synth.CONFIGS["4d_128x128x96"], parts, 8.0)
+    assert abs(c3["bytes_per_batched_iteration"] / 1e9 - 101.3) < 0.05
+    # rate 8 bin-iterations/s = one batched iteration per second
+    assert abs(c3["gbs"] - c3["bytes_per_batched_iteration"] / 1e9) < 1e-6

@@ -183,6 +183,63 @@ def test_graph_replays_between_eager_work(pkg, orc, small_dcf):
     assert rel_l2(x[0], xr) < CG_TOL
 
 
+def test_graph_follows_image_range_and_plain_solves(pkg, orc, small_dcf):
+    """A captured batched iteration is not replayed with settings it was
+    captured without: after a change of the reduceFOV box (Reco.cpp:626-671)
+    and a new prepare, the batched solve equals the plain one and the oracle
+    with that box; a plain cg_sense_images solve of another length in between
+    leaves cg_results sized by the current residual history."""
+    P = small_dcf
+    o = P["oracle"]
+    ctx = gpu_context(pkg, P)
+    ctx.set_bin_data(0, P["data"], P["dcf"])
+    print("image range: capture with the full grid", flush=True)
+    ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=5)            # captures the iteration graph
+    rng, off = (16, 20, 10), (1, -2, 0)
+    ctx.set_img_range(rng, off)
+    o.set_img_range(rng, off)
+    try:
+        print("image range: batched and plain solves with the crop box", flush=True)
+        xb, resb, itb = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=5)
+        xp, resp, itp = ctx.cg_sense_bins(mode=0, thresh=0.0, max_iter=5)
+        xr, _, itr = o.cg_sense(P["data"], thresh=0.0, max_iter=5, prec=64)
+    finally:
+        o.set_img_range(P["grid"])
+    assert itb[0] == itp[0] == itr == 6
+    assert rel_l2(xb[0], xp[0]) < 1e-6
+    assert rel_l2(xb[0], xr) < CG_TOL
+    # a plain solve from coil images with a longer history, then results of it
+    ctx.set_img_range(P["grid"], (0, 0, 0))
+    dd = P["data"] * P["dcf"][None, :]
+    img = ctx.nufft_bwd(dd)
+    print("image range: cg_sense_images then cg_results", flush=True)
+    x1, res1, it1 = ctx.cg_sense_images(img, thresh=0.0, max_iter=8)
+    assert ctx.cg_history_length() == 9
+    x2, res2, its2 = ctx.cg_results()
+    assert res2.shape == (1, 9) and its2[0] == it1 == 9
+    np.testing.assert_array_equal(res2[0], res1)
+    np.testing.assert_array_equal(x2[0], x1)
+    # the batched solve after the plain one captures again with its own settings
+    xb2, _, itb2 = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=5)
+    xr2, _, _ = o.cg_sense(P["data"], thresh=0.0, max_iter=5, prec=64)
+    assert itb2[0] == 6 and rel_l2(xb2[0], xr2) < CG_TOL
+
+
+def test_density_weights_of_the_wrong_length_are_refused(pkg, orc, small):
+    """The C side reads a fixed count of weights: the binding checks it."""
+    P = small
+    ctx = gpu_context(pkg, P)
+    ns = int(np.prod(P["samp"]))
+    with pytest.raises(ValueError):
+        ctx.set_density(np.ones(ns - 1, np.float32))
+    with pytest.raises(ValueError):
+        ctx.set_bin_data(0, P["data"], np.ones(10, np.float32))
+    idx = np.flatnonzero(P["data"][0] != 0).astype(np.int32)
+    with pytest.raises(ValueError):
+        ctx.set_bin_samples(0, idx, P["data"][:, idx], np.ones(ns, np.float32))
+    ctx.set_bin_samples(0, idx, P["data"][:, idx], np.ones(idx.size, np.float32))
+
+
 def test_cg_threshold_stops_like_reference(pkg, orc, small):
     """Loop control of Reco.cpp:230, 308-309 with an absolute threshold."""
     P = small

@@ -331,6 +331,35 @@ def test_fp32_reference_mode_close_to_fp64(orc):
     assert rel_l2(x32, x64) < 1e-4
 
 
+def test_atomic_coil_sum_is_the_same_sum(orc):
+    """The reference's loop shape for the coil sum (float atomics, thread
+    order, Reco.cpp:120-129, 257-262) adds the same terms as the fixed coil
+    order: one application agrees to float rounding, in FP64 exactly up to
+    the order of C terms; the CG image stays within 1e-4 of FP64; the
+    per-iteration times cover the loop."""
+    from problems import small_problem
+
+    P = small_problem(orc, nthreads=4)
+    o = P["oracle"]
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(o.ng) + 1j * rng.standard_normal(o.ng)
+    q64 = o.apply_normal(p, prec=64)
+    q32 = o.apply_normal(p.astype(np.complex64), prec=32)
+    x64, _, _ = o.cg_sense(P["data"], 0.0, 9, prec=64)
+    o.set_atomic_coil_sum(True)
+    try:
+        qa64 = o.apply_normal(p, prec=64)
+        qa32 = o.apply_normal(p.astype(np.complex64), prec=32)
+        xa, _, ita, loop_s, per = o.cg_sense(P["data"], 0.0, 9, prec=32, timed=True, iter_times=True)
+    finally:
+        o.set_atomic_coil_sum(False)
+    assert rel_l2(qa64, q64) < 1e-14
+    assert rel_l2(qa32, q32) < 1e-6
+    assert rel_l2(xa, x64) < 1e-4
+    assert ita == 10 and per.shape == (10,) and np.all(per > 0)
+    assert abs(per.sum() - loop_s) <= 1e-6 + 1e-3 * loop_s
+
+
 def test_exclude_missing_data_drops_zero_samples(orc):
     from problems import small_problem
 
