@@ -642,7 +642,7 @@ int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const 
     ZArgs z{};
     z.g = h->d_grid; z.tw = h->d_tw[2]; z.p = p; z.maps = p ? h->d_maps : nullptr;
     z.kft = deapod ? h->d_kft : nullptr; z.C = h->C; z.ng = h->ng; z.nxy = (long long)h->N[0] * h->N[1];
-    z.xp = h->xp; z.bp = h->bp; z.ny = h->N[1]; z.done = done;
+    z.xp = h->xp; z.bp = h->bp; z.ny = h->N[1]; z.done = done; z.nb = nb;
     {
         Prof t(h, "fft_z_fwd");
         CK(launch_fft_zpre(h->N[2], W, z, (long long)nb * z.nxy, h->st));
@@ -695,7 +695,7 @@ int run_inv_fft(cgs_ctx* h, int b0, int nb, float2* q, const float2* p, double* 
     z.q = q; z.part = part;
     z.scale = (float)(1e-3 / std::sqrt((double)h->ng));
     z.C = h->C; z.ng = h->ng; z.nxy = (long long)h->N[0] * h->N[1]; z.ny = h->N[1]; z.done = done;
-    z.xp = h->xp; z.bp = h->bp;
+    z.xp = h->xp; z.bp = h->bp; z.nb = nb;
     bool crop = false;
     for (int a = 0; a < 3; ++a) {
         if (h->imgRange[a] != h->N[a]) crop = true;

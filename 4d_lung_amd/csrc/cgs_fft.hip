@@ -630,6 +630,20 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_z1(float2* g, long long nli
     }
 }
 
+// Workgroup order of the z passes.  The bins of one launch share the coil
+// maps and the deapodization line of every (x, y) cell line, and the device
+// hands consecutive workgroups round-robin to its 8 XCDs (each with its own
+// L2).  Workgroup k therefore takes cell line xy = (s / nb) * 8 + k % 8 of
+// bin s % nb, s = k / 8: the nb workgroups of one cell line have ids 8 apart,
+// run on one XCD close together in time, and all but the first find that
+// line's maps in L2.  A bijection onto bin * nxy + xy whenever nxy is a
+// multiple of 8; otherwise (and for one bin) the bin-major order.
+__device__ __forceinline__ long long zpass_line(long long k, long long nxy, int nb) {
+    if (nb <= 1 || (nxy & 7)) return k;
+    const long long s = k >> 3;
+    return (s % nb) * nxy + (s / nb) * 8 + (k & 7);
+}
+
 // Forward z pass of E with the fused prologue.  Block = one (bin, x, y) line;
 // the block walks the coils in chunks of W, fetching the next chunk into
 // registers while the current one is transformed.
@@ -642,7 +656,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     __shared__ float2 twl[L];
     __shared__ float2 pline[HAS_P ? L : 1];   // p along the line (shared by all coils)
     __shared__ float kline[HAS_K ? L : 1];    // kernelFT along the line
-    const long long outer = blockIdx.x;             // bin * nxy + xy
+    const long long outer = zpass_line(blockIdx.x, a.nxy, a.nb);   // bin * nxy + xy
     const long long bin = outer / a.nxy, xy = outer % a.nxy;
     if (a.done && a.done[bin]) return;
     const long long cell0 = xy * L;                  // cell index of (x, y, z=0)
@@ -731,7 +745,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
     __shared__ float2 twl[L];
     __shared__ float kline[L];      // kernelFT * scale along the line
     __shared__ double red[2 * (NT / 64)];
-    const long long outer = blockIdx.x;
+    const long long outer = zpass_line(blockIdx.x, a.nxy, a.nb);   // bin * nxy + xy
     const long long bin = outer / a.nxy, xy = outer % a.nxy;
     if (a.done && a.done[bin]) return;
     const long long cell0 = xy * L;
@@ -831,8 +845,8 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
         if (threadIdx.x == 0) {
             double sr = 0.0, si = 0.0;
             for (int w = 0; w < NT / 64; ++w) { sr += red[2 * w]; si += red[2 * w + 1]; }
-            a.part[2 * blockIdx.x] = sr;
-            a.part[2 * blockIdx.x + 1] = si;
+            a.part[2 * outer] = sr;
+            a.part[2 * outer + 1] = si;
         }
     }
 }

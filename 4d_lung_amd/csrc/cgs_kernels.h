@@ -45,6 +45,7 @@ struct ZArgs {
     int crop;              // reduceFOV active
     int lo[3], hi[3];      // kept box, inclusive on both ends
     const int* done;       // per-bin flags of the launch's bins, or null
+    int nb;                // bins of the launch (workgroup order, zpass_line)
 };
 
 bool fft_len_supported(int L);
