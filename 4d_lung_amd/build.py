@@ -36,6 +36,7 @@ SOURCES = [
     ("cgs_setup.hip", ["-ffp-contract=off"]),
     ("cgs_voronoi.hip", ["-ffp-contract=off"]),
     ("cgs_api.hip", ["-ffp-contract=off"]),
+    ("cgs_noalloc.hip", ["-ffp-contract=off"]),
 ]
 
 

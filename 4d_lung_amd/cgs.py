@@ -95,6 +95,7 @@ def lib():
         "cgs_fft_destroy": (I, [P]),
         "cgs_fft_forward": (I, [P, P]),
         "cgs_fft_backward": (I, [P, P, I]),
+        "cgs_fft_device_noalloc": (I, [I3, P, I, I, P]),
         "cgs_profile_enable": (I, [P, I]),
         "cgs_profile_report": (I, [P, ctypes.c_char_p, I]),
         "cgs_profile_reset": (I, [P]),
@@ -424,3 +425,15 @@ class Fourier:
         if st:
             raise CgsError(st, "transformBWD failed")
         return a
+
+
+def fft_device_noalloc(dims, a_ptr, direction, rescale=True, stream=None):
+    """Centred 3D FFT in place on a device array at address ``a_ptr``
+    (complex64, ``dims`` = (Nx, Ny, Nz)) on ``stream`` (a hipStream_t
+    handle as an int, or None), without any device allocation: direction -1
+    is Fourier::transformFWD, +1 transformBWD (with ``rescale``).  Returns
+    without synchronising."""
+    st = lib().cgs_fft_device_noalloc(_i3(dims), ctypes.c_void_p(int(a_ptr)), int(direction), int(bool(rescale)),
+                                      None if stream is None else ctypes.c_void_p(int(stream)))
+    if st:
+        raise CgsError(st, "cgs_fft_device_noalloc failed for dims %s" % (tuple(dims),))

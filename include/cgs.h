@@ -203,6 +203,17 @@ int cgs_fft_forward(cgs_fft* f, float* a);
 /* Fourier::transformBWD (Fourier.cpp:26-42), in place on a host array. */
 int cgs_fft_backward(cgs_fft* f, float* a, int rescale);
 
+/* Centred 3D FFT of one complex grid held in device memory, in place, on
+ * `stream` (a hipStream_t; NULL = the default stream): dir -1 is
+ * Fourier::transformFWD, dir +1 Fourier::transformBWD with rescale as there
+ * (Fourier.cpp:19-42).  Allocates nothing: the twiddle tables live in a fixed
+ * 12 KiB device array that each GPU holds for the life of the process, so
+ * callers that cannot allocate (code under HIP graph capture, serving loops,
+ * harnesses that forbid allocation) can use it.  Every dimension must be a
+ * supported length of at most 512 (else CGS_ERR_ARG).  Calls that overlap on
+ * one GPU share that array and race.  No counterpart in the reference. */
+int cgs_fft_device_noalloc(const int dims[3], float* a_dev, int dir, int rescale, void* stream);
+
 /* ---- timing ---- */
 /* Per-kernel HIP-event timing on the context's stream (off by default). */
 int cgs_profile_enable(cgs_ctx* h, int on);
