@@ -90,6 +90,8 @@ def lib():
         "cgs_kernel_info": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I), ctypes.POINTER(D)]),
         "cgs_voronoi_volumes": (I, [P, LL, P]),
         "cgs_voronoi_density": (I, [P, LL, I, I, D, P]),
+        "cgs_voronoi_volumes_rule": (I, [P, LL, I, P]),
+        "cgs_voronoi_density_rule": (I, [P, LL, I, I, I, D, P]),
         "cgs_set_bin_voronoi": (I, [P, I]),
         "cgs_fft_create": (I, [I3, ctypes.POINTER(P)]),
         "cgs_fft_destroy": (I, [P]),
@@ -375,23 +377,30 @@ class Context:
         return int(self._L.cgs_device_bytes(self.h))
 
 
-def voronoi_volumes(points):
-    """Voronoi cell volumes on the device (0 for unbounded cells)."""
+VORONOI_RULES = {"hull": 0, "clipped": 1}
+
+
+def voronoi_volumes(points, rule="hull"):
+    """Voronoi cell volumes on the device.  rule "hull" (the reference's,
+    functions.h:148-150): 0 only for points on the convex hull; "clipped":
+    0 also for cells reaching more than one hash cell beyond the points'
+    bounding box (the per-bin weights of respiratory bins)."""
     p = np.ascontiguousarray(points, dtype=np.float64)
     vol = np.zeros(len(p), np.float64)
-    st = lib().cgs_voronoi_volumes(_ptr(p), len(p), _ptr(vol))
+    st = lib().cgs_voronoi_volumes_rule(_ptr(p), len(p), VORONOI_RULES[rule], _ptr(vol))
     if st:
-        raise CgsError(st, "cgs_voronoi_volumes failed")
+        raise CgsError(st, "cgs_voronoi_volumes_rule failed")
     return vol
 
 
-def voronoi_density(points, average=False, sx=0, cutoff=1.0):
+def voronoi_density(points, average=False, sx=0, cutoff=1.0, rule="hull"):
     """estimateDensityV3D (functions.h:161-258) on the device."""
     p = np.ascontiguousarray(points, dtype=np.float64)
     d = np.zeros(len(p), np.float32)
-    st = lib().cgs_voronoi_density(_ptr(p), len(p), int(bool(average)), int(sx), float(cutoff), _ptr(d))
+    st = lib().cgs_voronoi_density_rule(_ptr(p), len(p), VORONOI_RULES[rule], int(bool(average)), int(sx),
+                                        float(cutoff), _ptr(d))
     if st:
-        raise CgsError(st, "cgs_voronoi_density failed")
+        raise CgsError(st, "cgs_voronoi_density_rule failed")
     return d
 
 

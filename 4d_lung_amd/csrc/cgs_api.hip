@@ -1433,57 +1433,14 @@ int cgs_profile_report(cgs_ctx* h, char* buf, int len) {
 long long cgs_device_bytes(const cgs_ctx* h) { return h ? h->dev_bytes : 0; }
 
 // ------------------------------------------------------------------ Voronoi
-static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, double* vol_host, hipStream_t st) {
-    if (np < 2) {
-        for (long long i = 0; i < np; i++) vol_host[i] = 0.0;
-        return CGS_OK;
-    }
-    // uniform hash: about six points per cell, ascending point index per cell
-    double lo[3], hi[3];
-    for (int a = 0; a < 3; a++) { lo[a] = pts[a]; hi[a] = pts[a]; }
-    for (long long i = 0; i < np; i++)
-        for (int a = 0; a < 3; a++) {
-            lo[a] = std::fmin(lo[a], pts[3 * i + a]);
-            hi[a] = std::fmax(hi[a], pts[3 * i + a]);
-        }
-    double vol = 1.0;
-    for (int a = 0; a < 3; a++) vol *= std::fmax(hi[a] - lo[a], 1e-12);
-    VorHash H{};
-    H.h = std::cbrt(vol * 6.0 / (double)np);
-    for (int a = 0; a < 3; a++) { H.lo[a] = lo[a]; H.n[a] = (int)((hi[a] - lo[a]) / H.h) + 1; }
-    for (int a = 0; a < 3; a++) { H.blo[a] = lo[a] - H.h; H.bhi[a] = hi[a] + H.h; }
-    const long long nc = (long long)H.n[0] * H.n[1] * H.n[2];
-    std::vector<int> start(nc + 1, 0), key(np), idx(np);
-    for (long long i = 0; i < np; i++) {
-        int c3[3];
-        for (int a = 0; a < 3; a++) {
-            int k = (int)((pts[3 * i + a] - H.lo[a]) / H.h);
-            c3[a] = k < 0 ? 0 : (k >= H.n[a] ? H.n[a] - 1 : k);
-        }
-        key[i] = (c3[0] * H.n[1] + c3[1]) * H.n[2] + c3[2];
-        start[key[i] + 1]++;
-    }
-    for (long long c = 0; c < nc; c++) start[c + 1] += start[c];
-    std::vector<int> cur(start.begin(), start.end() - 1);
-    for (long long i = 0; i < np; i++) idx[cur[key[i]]++] = (int)i;
-
-    double *dp = nullptr, *dv = nullptr;
-    int *ds = nullptr, *di = nullptr, *dov = nullptr, *redo = nullptr;
-    CKS(dalloc(h, &dp, (size_t)np * 3));
-    CKS(dalloc(h, &dv, (size_t)np));
-    CKS(dalloc(h, &ds, (size_t)nc + 1));
-    CKS(dalloc(h, &di, (size_t)np));
-    int* redo2 = nullptr;
-    CKS(dalloc(h, &dov, 3));          // [0] cells to rebuild, [1] final overflows, [2] rebuilt again
-    CKS(dalloc(h, &redo, (size_t)np));
-    CKS(dalloc(h, &redo2, (size_t)np));
-    CK(hipMemcpy(dp, pts, sizeof(double) * 3 * np, hipMemcpyHostToDevice));
-    CK(hipMemcpy(ds, start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
-    CK(hipMemcpy(di, idx.data(), sizeof(int) * np, hipMemcpyHostToDevice));
-    CK(hipMemset(dov, 0, 3 * sizeof(int)));
-    H.start = ds;
-    H.idx = di;
-    CK(launch_voronoi(dp, np, H, dv, redo, dov, st));
+// One build of the listed cells (all when list is null; wide: from the wide
+// box of the hull rule): small limits in LDS, short overflow lists again with
+// medium limits in LDS, the rest with large limits in device memory.  Returns
+// the closed cells that outgrew the large limits in *over.
+static int voronoi_pass(cgs_ctx* h, const double* dp, long long n, const int* list, bool wide, const VorHash& H,
+                        double* dv, int* redo, int* redo2, int* dov, int* over, hipStream_t st) {
+    CK(hipMemsetAsync(dov, 0, 3 * sizeof(int), st));
+    CK(launch_voronoi(dp, n, list, wide, H, dv, redo, dov, st));
     int cnt[3] = {0, 0, 0};
     CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
@@ -1492,7 +1449,7 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
     // thin cells of sparse line sets) finish sooner in the last pass, which
     // builds every listed cell at once from device memory.
     if (cnt[0] > 0 && cnt[0] <= 6144) {
-        CK(launch_voronoi_mid(dp, H, redo, cnt[0], dv, redo2, dov + 2, st));
+        CK(launch_voronoi_mid(dp, wide, H, redo, cnt[0], dv, redo2, dov + 2, st));
         CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
         std::swap(redo, redo2);
@@ -1512,15 +1469,91 @@ static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, doub
         char *cells = nullptr, *scr = nullptr;
         CKS(dalloc(h, &cells, cell_b * nthreads));
         CKS(dalloc(h, &scr, scr_b * nthreads));
-        CK(launch_voronoi_large(dp, H, redo, cnt[0], nthreads, cells, scr, dv, dov + 1, st));
+        CK(launch_voronoi_large(dp, wide, H, redo, cnt[0], nthreads, cells, scr, dv, dov + 1, st));
         CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
         dfree(h, cells, cell_b * nthreads);
         dfree(h, scr, scr_b * nthreads);
     }
-    const int over = cnt[1];
+    *over += cnt[1];
+    return CGS_OK;
+}
+
+// Voronoi cell volumes of np points.  rule CGS_VORONOI_HULL: 0 only for
+// unbounded cells (the reference); cells that keep a face of the first box
+// are built again from the wide box.  CGS_VORONOI_CLIPPED: 0 for every cell
+// that keeps a face of the first box.
+static int voronoi_volumes_dev(cgs_ctx* h, const double* pts, long long np, int rule, double* vol_host,
+                               hipStream_t st) {
+    if (np < 2) {
+        for (long long i = 0; i < np; i++) vol_host[i] = 0.0;
+        return CGS_OK;
+    }
+    // uniform hash: about six points per cell, ascending point index per cell
+    double lo[3], hi[3];
+    for (int a = 0; a < 3; a++) { lo[a] = pts[a]; hi[a] = pts[a]; }
+    for (long long i = 0; i < np; i++)
+        for (int a = 0; a < 3; a++) {
+            lo[a] = std::fmin(lo[a], pts[3 * i + a]);
+            hi[a] = std::fmax(hi[a], pts[3 * i + a]);
+        }
+    double vol = 1.0;
+    for (int a = 0; a < 3; a++) vol *= std::fmax(hi[a] - lo[a], 1e-12);
+    VorHash H{};
+    H.h = std::cbrt(vol * 6.0 / (double)np);
+    for (int a = 0; a < 3; a++) { H.lo[a] = lo[a]; H.n[a] = (int)((hi[a] - lo[a]) / H.h) + 1; }
+    for (int a = 0; a < 3; a++) { H.blo[a] = lo[a] - H.h; H.bhi[a] = hi[a] + H.h; }
+    double ext = 0.0;
+    for (int a = 0; a < 3; a++) ext = std::fmax(ext, hi[a] - lo[a]);
+    const double wide = kVorWideBox * ext;
+    for (int a = 0; a < 3; a++) { H.wlo[a] = lo[a] - wide; H.whi[a] = hi[a] + wide; }
+    const long long nc = (long long)H.n[0] * H.n[1] * H.n[2];
+    std::vector<int> start(nc + 1, 0), key(np), idx(np);
+    for (long long i = 0; i < np; i++) {
+        int c3[3];
+        for (int a = 0; a < 3; a++) {
+            int k = (int)((pts[3 * i + a] - H.lo[a]) / H.h);
+            c3[a] = k < 0 ? 0 : (k >= H.n[a] ? H.n[a] - 1 : k);
+        }
+        key[i] = (c3[0] * H.n[1] + c3[1]) * H.n[2] + c3[2];
+        start[key[i] + 1]++;
+    }
+    for (long long c = 0; c < nc; c++) start[c + 1] += start[c];
+    std::vector<int> cur(start.begin(), start.end() - 1);
+    for (long long i = 0; i < np; i++) idx[cur[key[i]]++] = (int)i;
+
+    double *dp = nullptr, *dv = nullptr;
+    int *ds = nullptr, *di = nullptr, *dov = nullptr, *redo = nullptr, *redo2 = nullptr, *dl = nullptr;
+    CKS(dalloc(h, &dp, (size_t)np * 3));
+    CKS(dalloc(h, &dv, (size_t)np));
+    CKS(dalloc(h, &ds, (size_t)nc + 1));
+    CKS(dalloc(h, &di, (size_t)np));
+    CKS(dalloc(h, &dov, 3));          // [0] cells to rebuild, [1] final overflows, [2] rebuilt again
+    CKS(dalloc(h, &redo, (size_t)np));
+    CKS(dalloc(h, &redo2, (size_t)np));
+    CK(hipMemcpy(dp, pts, sizeof(double) * 3 * np, hipMemcpyHostToDevice));
+    CK(hipMemcpy(ds, start.data(), sizeof(int) * (nc + 1), hipMemcpyHostToDevice));
+    CK(hipMemcpy(di, idx.data(), sizeof(int) * np, hipMemcpyHostToDevice));
+    H.start = ds;
+    H.idx = di;
+    int over = 0;
+    CKS(voronoi_pass(h, dp, np, nullptr, false, H, dv, redo, redo2, dov, &over, st));
     CK(hipMemcpyAsync(vol_host, dv, sizeof(double) * np, hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
+    if (rule == CGS_VORONOI_HULL) {
+        // the cells still open in the first box, built again from the wide one
+        std::vector<int> open;
+        for (long long i = 0; i < np; i++)
+            if (vol_host[i] == 0.0) open.push_back((int)i);
+        if (!open.empty()) {
+            CKS(dalloc(h, &dl, open.size()));
+            CK(hipMemcpy(dl, open.data(), sizeof(int) * open.size(), hipMemcpyHostToDevice));
+            CKS(voronoi_pass(h, dp, (long long)open.size(), dl, true, H, dv, redo, redo2, dov, &over, st));
+            CK(hipMemcpyAsync(vol_host, dv, sizeof(double) * np, hipMemcpyDeviceToHost, st));
+            CK(hipStreamSynchronize(st));
+            dfree(h, dl, open.size());
+        }
+    }
     dfree(h, dp, (size_t)np * 3); dfree(h, dv, (size_t)np); dfree(h, ds, (size_t)nc + 1);
     dfree(h, di, (size_t)np); dfree(h, dov, 3); dfree(h, redo, (size_t)np); dfree(h, redo2, (size_t)np);
     if (over) return fail(h, CGS_ERR_ARG, std::to_string(over) + " Voronoi cells exceeded the face/vertex limits");
@@ -1547,18 +1580,27 @@ static void normalise_density(const double* vol, long long np, int average, int 
     for (long long i = 0; i < np; i++) dcf[i] = (float)(cv[i] * mx);
 }
 
-int cgs_voronoi_volumes(const double* pts, long long np, double* vol) {
-    if (!pts || !vol || np < 0) return CGS_ERR_ARG;
-    return voronoi_volumes_dev(nullptr, pts, np, vol, nullptr);
+int cgs_voronoi_volumes_rule(const double* pts, long long np, int rule, double* vol) {
+    if (!pts || !vol || np < 0 || (rule != CGS_VORONOI_HULL && rule != CGS_VORONOI_CLIPPED)) return CGS_ERR_ARG;
+    return voronoi_volumes_dev(nullptr, pts, np, rule, vol, nullptr);
 }
 
-int cgs_voronoi_density(const double* pts, long long np, int average, int sx, double cutoff, float* dcf) {
-    if (!pts || !dcf || np < 1) return CGS_ERR_ARG;
+int cgs_voronoi_density_rule(const double* pts, long long np, int rule, int average, int sx, double cutoff,
+                             float* dcf) {
+    if (!pts || !dcf || np < 1 || (rule != CGS_VORONOI_HULL && rule != CGS_VORONOI_CLIPPED)) return CGS_ERR_ARG;
     std::vector<double> vol(np);
-    const int st = voronoi_volumes_dev(nullptr, pts, np, vol.data(), nullptr);
+    const int st = voronoi_volumes_dev(nullptr, pts, np, rule, vol.data(), nullptr);
     if (st != CGS_OK) return st;
     normalise_density(vol.data(), np, average, sx, cutoff, dcf);
     return CGS_OK;
+}
+
+int cgs_voronoi_volumes(const double* pts, long long np, double* vol) {
+    return cgs_voronoi_volumes_rule(pts, np, CGS_VORONOI_HULL, vol);
+}
+
+int cgs_voronoi_density(const double* pts, long long np, int average, int sx, double cutoff, float* dcf) {
+    return cgs_voronoi_density_rule(pts, np, CGS_VORONOI_HULL, average, sx, cutoff, dcf);
 }
 
 int cgs_set_bin_voronoi(cgs_ctx* h, int bin) {
@@ -1576,7 +1618,7 @@ int cgs_set_bin_voronoi(cgs_ctx* h, int bin) {
     for (long long s = 0; s < B.m; s++)
         for (int a = 0; a < 3; a++) pts[3 * s + a] = traj[3 * (size_t)act[s] + a];
     std::vector<double> vol(B.m);
-    CKS(voronoi_volumes_dev(h, pts.data(), B.m, vol.data(), h->st));
+    CKS(voronoi_volumes_dev(h, pts.data(), B.m, CGS_VORONOI_CLIPPED, vol.data(), h->st));
     std::vector<float> w(B.m), full(h->ns, 0.0f);
     normalise_density(vol.data(), B.m, 0, 0, 1.0, w.data());
     for (long long s = 0; s < B.m; s++) full[act[s]] = w[s];

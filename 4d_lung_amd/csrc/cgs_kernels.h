@@ -185,23 +185,32 @@ hipError_t launch_scale_d(float2* a, long long n, double f, hipStream_t s);
 struct VorHash {
     double lo[3];
     double blo[3], bhi[3];   // bounding box of the points widened by one hash cell
+    double wlo[3], whi[3];   // bounding box widened by kVorWideBox times the largest extent
     double h;
     int n[3];
     const int* start;
     const int* idx;
 };
 
-// First pass (small cell limits): volumes of all points; points whose cell
-// outgrew the limits are appended to redo[*nredo].
-hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* redo, int* nredo,
-                          hipStream_t s);
-// Second pass (large limits) over redo[0..nredo) with nthreads threads (a
+// Hull rule (the reference's: 0 only for unbounded cells): cells that keep a
+// face of the first box are built again from a box wider than the points by
+// this factor times their largest extent, visiting every hash cell.
+constexpr double kVorWideBox = 65536.0;
+
+// First pass (small cell limits): volumes of the listed points (list[0..n),
+// or every point when list is null); points whose cell outgrew the limits are
+// appended to redo[*nredo].  wide: build from the wide box (hull rule, second
+// stage).
+hipError_t launch_voronoi(const double* pts, long long n, const int* list, bool wide, const VorHash& H, double* vol,
+                          int* redo, int* nredo, hipStream_t s);
+// Middle pass (medium limits) over redo[0..nredo); overflows go to redo2.
+hipError_t launch_voronoi_mid(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo, double* vol,
+                              int* redo2, int* nredo2, hipStream_t s);
+// Last pass (large limits) over redo[0..nredo) with nthreads threads (a
 // multiple of 64), each using one cell and one scratch slot of the
 // workspaces; closed cells that still overflow are counted in *overflow.
-hipError_t launch_voronoi_mid(const double* pts, const VorHash& H, const int* redo, int nredo, double* vol,
-                              int* redo2, int* nredo2, hipStream_t s);
 void voronoi_large_sizes(size_t* cell, size_t* scratch);
-hipError_t launch_voronoi_large(const double* pts, const VorHash& H, const int* redo, int nredo, int nthreads,
-                                void* cells, void* scratch, double* vol, int* overflow, hipStream_t s);
+hipError_t launch_voronoi_large(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo,
+                                int nthreads, void* cells, void* scratch, double* vol, int* overflow, hipStream_t s);
 
 }  // namespace cgs

@@ -289,9 +289,11 @@ __device__ double cell_volume(const Cell<LM>& c) {
 // has a box face when it outgrows them is taken as unbounded (0, overflow
 // cleared) and a closed cell's overflow stays set; with the first (small)
 // limits any overflow is left set so that the cell is built again.
+// wide = false: the box of the points widened by one hash cell, at most RMAX
+// shells; wide = true (hull rule, second stage): the wide box, every shell.
 template <class LM>
 __device__ double point_volume(const double* __restrict__ p, long long i, const VorHash& H, Cell<LM>& c,
-                               ClipScratch<LM>& w, bool final_pass) {
+                               ClipScratch<LM>& w, bool final_pass, bool wide) {
     const V3 pi = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     int c3[3];
     for (int a = 0; a < 3; a++) {
@@ -299,9 +301,15 @@ __device__ double point_volume(const double* __restrict__ p, long long i, const 
         c3[a] = k < 0 ? 0 : (k >= H.n[a] ? H.n[a] - 1 : k);
     }
     double blo[3], bhi[3];
-    for (int a = 0; a < 3; a++) { blo[a] = H.blo[a] - p[3 * i + a]; bhi[a] = H.bhi[a] - p[3 * i + a]; }
+    for (int a = 0; a < 3; a++) {
+        blo[a] = (wide ? H.wlo[a] : H.blo[a]) - p[3 * i + a];
+        bhi[a] = (wide ? H.whi[a] : H.bhi[a]) - p[3 * i + a];
+    }
     init_box(c, blo, bhi);
-    for (int r = 0; r <= RMAX; r++) {
+    int rlim = RMAX;
+    if (wide)
+        for (int a = 0; a < 3; a++) rlim = H.n[a] > rlim ? H.n[a] : rlim;
+    for (int r = 0; r <= rlim; r++) {
         for (int dx = -r; dx <= r; dx++)
             for (int dy = -r; dy <= r; dy++)
                 for (int dz = -r; dz <= r; dz++) {
@@ -341,14 +349,16 @@ __device__ double point_volume(const double* __restrict__ p, long long i, const 
 // dependent, data-dependent accesses to the cell, which LDS serves far faster
 // than private memory.  Grid-stride over the points; cells that outgrow the
 // limits are listed for the second pass.
-__global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, long long np, VorHash H,
+__global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, long long n,
+                                               const int* __restrict__ list, bool wide, VorHash H,
                                                double* __restrict__ vol, int* __restrict__ redo,
                                                int* __restrict__ nredo) {
     __shared__ Cell<SmallLimits> c;
     __shared__ ClipScratch<SmallLimits> w;
     if (threadIdx.x != 0) return;
-    for (long long i = blockIdx.x; i < np; i += gridDim.x) {
-        vol[i] = point_volume(p, i, H, c, w, false);
+    for (long long k = blockIdx.x; k < n; k += gridDim.x) {
+        const long long i = list ? (long long)list[k] : k;
+        vol[i] = point_volume(p, i, H, c, w, false, wide);
         if (c.overflow) redo[atomicAdd(nredo, 1)] = (int)i;
     }
 }
@@ -356,7 +366,7 @@ __global__ __launch_bounds__(64) void k_voronoi(const double* __restrict__ p, lo
 // Middle pass over the listed points with the medium limits, again one cell
 // per wave in LDS (about 32 KB); cells that outgrow these too are listed for
 // the last pass.
-__global__ __launch_bounds__(64) void k_voronoi_mid(const double* __restrict__ p, VorHash H,
+__global__ __launch_bounds__(64) void k_voronoi_mid(const double* __restrict__ p, bool wide, VorHash H,
                                                    const int* __restrict__ redo, int nredo,
                                                    double* __restrict__ vol, int* __restrict__ redo2,
                                                    int* __restrict__ nredo2) {
@@ -365,7 +375,7 @@ __global__ __launch_bounds__(64) void k_voronoi_mid(const double* __restrict__ p
     if (threadIdx.x != 0) return;
     for (long long k = blockIdx.x; k < nredo; k += gridDim.x) {
         const long long i = redo[k];
-        vol[i] = point_volume(p, i, H, c, w, false);
+        vol[i] = point_volume(p, i, H, c, w, false, wide);
         if (c.overflow) redo2[atomicAdd(nredo2, 1)] = (int)i;
     }
 }
@@ -373,7 +383,7 @@ __global__ __launch_bounds__(64) void k_voronoi_mid(const double* __restrict__ p
 // Last pass over the listed points with the large limits; each thread's
 // cell and working arrays live in a device workspace (too large for private
 // memory).  Closed cells that outgrow these limits as well are counted.
-__global__ __launch_bounds__(64) void k_voronoi_large(const double* __restrict__ p, VorHash H,
+__global__ __launch_bounds__(64) void k_voronoi_large(const double* __restrict__ p, bool wide, VorHash H,
                                                      const int* __restrict__ redo, int nredo,
                                                      Cell<LargeLimits>* __restrict__ cells,
                                                      ClipScratch<LargeLimits>* __restrict__ scratch,
@@ -383,27 +393,28 @@ __global__ __launch_bounds__(64) void k_voronoi_large(const double* __restrict__
     ClipScratch<LargeLimits>& w = scratch[t];
     for (long long k = t; k < nredo; k += (long long)gridDim.x * blockDim.x) {
         const long long i = redo[k];
-        vol[i] = point_volume(p, i, H, c, w, true);
+        vol[i] = point_volume(p, i, H, c, w, true, wide);
         if (c.overflow) atomicAdd(overflow, 1);
     }
 }
 
 }  // namespace vor
 
-hipError_t launch_voronoi(const double* pts, long long np, const VorHash& H, double* vol, int* redo, int* nredo,
-                          hipStream_t s) {
+hipError_t launch_voronoi(const double* pts, long long n, const int* list, bool wide, const VorHash& H, double* vol,
+                          int* redo, int* nredo, hipStream_t s) {
     // one wave per cell in flight; 8192 workgroups cover every CU several
     // times over and grid-stride over the rest
-    const long long blocks = std::min<long long>(np, 8192);
+    const long long blocks = std::min<long long>(n, 8192);
     if (blocks == 0) return hipSuccess;
-    vor::k_voronoi<<<(unsigned)blocks, 64, 0, s>>>(pts, np, H, vol, redo, nredo);
+    vor::k_voronoi<<<(unsigned)blocks, 64, 0, s>>>(pts, n, list, wide, H, vol, redo, nredo);
     return hipGetLastError();
 }
 
-hipError_t launch_voronoi_mid(const double* pts, const VorHash& H, const int* redo, int nredo, double* vol,
+hipError_t launch_voronoi_mid(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo, double* vol,
                               int* redo2, int* nredo2, hipStream_t s) {
     if (nredo == 0) return hipSuccess;
-    vor::k_voronoi_mid<<<(unsigned)std::min(nredo, 8192), 64, 0, s>>>(pts, H, redo, nredo, vol, redo2, nredo2);
+    vor::k_voronoi_mid<<<(unsigned)std::min(nredo, 8192), 64, 0, s>>>(pts, wide, H, redo, nredo, vol, redo2,
+                                                                     nredo2);
     return hipGetLastError();
 }
 
@@ -412,11 +423,11 @@ void voronoi_large_sizes(size_t* cell, size_t* scratch) {
     *scratch = sizeof(vor::ClipScratch<vor::LargeLimits>);
 }
 
-hipError_t launch_voronoi_large(const double* pts, const VorHash& H, const int* redo, int nredo, int nthreads,
-                                void* cells, void* scratch, double* vol, int* overflow, hipStream_t s) {
+hipError_t launch_voronoi_large(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo,
+                                int nthreads, void* cells, void* scratch, double* vol, int* overflow, hipStream_t s) {
     if (nredo == 0) return hipSuccess;
     vor::k_voronoi_large<<<(unsigned)(nthreads / 64), 64, 0, s>>>(
-        pts, H, redo, nredo, (vor::Cell<vor::LargeLimits>*)cells, (vor::ClipScratch<vor::LargeLimits>*)scratch, vol,
+        pts, wide, H, redo, nredo, (vor::Cell<vor::LargeLimits>*)cells, (vor::ClipScratch<vor::LargeLimits>*)scratch, vol,
         overflow);
     return hipGetLastError();
 }

@@ -15,7 +15,8 @@ double mean(const std::vector<float>& t) {
 
 // estimateDensityV3D (functions.h:161-258): cell volumes on the device
 // (cgs_voronoi_density: half-space clipping instead of a CGAL triangulation,
-// unbounded cells 0), averaging and normalisation as the reference.
+// with the reference's rule: 0 only for the unbounded cells of points on the
+// convex hull), averaging and normalisation as the reference.
 std::vector<float> estimateDensityV3D(const mat2d<double>& traj, const int& sX, const int& sY, const int& sZ,
                                       const bool& average, const double& cutoff) {
     const long long nSamples = (long long)sX * sY * sZ;

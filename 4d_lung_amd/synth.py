@@ -240,7 +240,8 @@ def binned_workload(pkg, cfg, log=None, pruning=-1):
     and the acquired lines are split into the seeded respiratory bins; every
     bin keeps its samples in compact form (original indices, ``C x m``
     values) and, for ``dcf="voronoi"``, the Voronoi weights of its own
-    samples (cgs_voronoi_density on the device).  Returns a dict with ``ctx``,
+    samples (cgs_voronoi_density_rule on the device with the clipped rule:
+    cells reaching beyond the bin's bounding box get 0).  Returns a dict with ``ctx``,
     ``traj``, ``maps``, ``labels`` and ``bins`` = [(idx, values, weights)] and
     ``voronoi_seconds`` (None without density weights).
     """
@@ -276,7 +277,7 @@ def binned_workload(pkg, cfg, log=None, pruning=-1):
     voronoi = cfg.get("dcf", "voronoi") == "voronoi"
     for b in range(B):
         idx = np.flatnonzero(sample_mask_from_lines(labels == b, samp[0])).astype(np.int32)
-        w = pkg.cgs.voronoi_density(traj[idx]) if voronoi else None
+        w = pkg.cgs.voronoi_density(traj[idx], rule="clipped") if voronoi else None
         bins.append((idx, np.ascontiguousarray(data[:, idx]), w))
         if voronoi:
             say("bin %d: %d samples, Voronoi weights after %.1fs" % (b, idx.size, time.time() - tv))

@@ -151,14 +151,35 @@ int cgs_bin_tables(cgs_ctx* h, int bin, int32_t* nearest, int32_t* perm, int32_t
 int cgs_kernel_info(const cgs_ctx* h, int* kw, int* ntiles, double* thr);
 
 /* ---- Voronoi density compensation (estimateDensityV3D, functions.h:161-258) ---- */
-/* Cell volumes of np points (np x 3 doubles), 0 for unbounded (hull) cells. */
+/* Rules for the cells whose Voronoi region reaches far out:
+ *   CGS_VORONOI_HULL     the reference's (functions.h:148-150): volume 0 only
+ *                        for unbounded cells, i.e. points on the convex hull;
+ *                        every other point gets its finite volume however
+ *                        large (the normalisation by the maximum then follows
+ *                        the largest one).
+ *   CGS_VORONOI_CLIPPED  extension for the per-bin weights of respiratory
+ *                        bins: cells that reach more than one hash cell
+ *                        (about the mean point spacing) beyond the points'
+ *                        bounding box get 0 too, so that the near-degenerate
+ *                        boundary cells of sparse line subsets (up to 1e26
+ *                        times the median volume) do not crush every other
+ *                        weight in the normalisation.  No reference
+ *                        counterpart: the reference computes one set of
+ *                        weights over all trajectory points. */
+#define CGS_VORONOI_HULL 0
+#define CGS_VORONOI_CLIPPED 1
+/* Cell volumes of np points (np x 3 doubles) by the given rule. */
+int cgs_voronoi_volumes_rule(const double* pts, long long np, int rule, double* vol);
+/* Weights = volume / max volume (functions.h:234-245) by the given rule; with
+ * average != 0 the per-index averaging of functions.h:212-232 (index i mod
+ * sx, cut-off) is applied first. */
+int cgs_voronoi_density_rule(const double* pts, long long np, int rule, int average, int sx, double cutoff,
+                             float* dcf);
+/* The same with the reference's rule (CGS_VORONOI_HULL). */
 int cgs_voronoi_volumes(const double* pts, long long np, double* vol);
-/* Weights = volume / max volume (functions.h:234-245); with average != 0 the
- * per-index averaging of functions.h:212-232 (index i mod sx, cut-off) is
- * applied first. */
 int cgs_voronoi_density(const double* pts, long long np, int average, int sx, double cutoff, float* dcf);
-/* Voronoi weights over the active samples of one bin become that bin's density
- * weights (per-bin DCF of the 4D workloads). */
+/* Voronoi weights (CGS_VORONOI_CLIPPED) over the active samples of one bin
+ * become that bin's density weights (per-bin DCF of the 4D workloads). */
 int cgs_set_bin_voronoi(cgs_ctx* h, int bin);
 
 /* ---- differentiable operators on device memory ----

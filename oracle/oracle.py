@@ -77,6 +77,11 @@ def lib():
         L.orc_voronoi_density.restype = ctypes.c_long
         L.orc_voronoi_density.argtypes = [P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_double, P,
                                           ctypes.c_int]
+        L.orc_voronoi_volumes_rule.restype = ctypes.c_long
+        L.orc_voronoi_volumes_rule.argtypes = [P, ctypes.c_long, ctypes.c_int, P, ctypes.c_int]
+        L.orc_voronoi_density_rule.restype = ctypes.c_long
+        L.orc_voronoi_density_rule.argtypes = [P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_double, P, ctypes.c_int]
         L.orc_density_from_volumes.restype = None
         L.orc_density_from_volumes.argtypes = [P, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_double, P]
         _lib = L
@@ -103,22 +108,29 @@ def i0_std(x):
     return lib().orc_i0_std(float(x))
 
 
-def voronoi_volumes(points, nthreads=0):
-    """Cell volumes by half-space clipping (the device algorithm on the CPU)."""
+VORONOI_RULES = {"hull": 0, "clipped": 1}
+
+
+def voronoi_volumes(points, nthreads=0, rule="hull"):
+    """Cell volumes by half-space clipping (the device algorithm on the CPU).
+    rule "hull": 0 only for unbounded cells (points on the convex hull), as
+    the reference (functions.h:148-150); "clipped": 0 also for cells that
+    reach more than one hash cell beyond the points' bounding box."""
     p = np.ascontiguousarray(points, dtype=np.float64)
     vol = np.zeros(len(p), np.float64)
-    over = lib().orc_voronoi_volumes(_ptr(p), len(p), _ptr(vol), int(nthreads))
+    over = lib().orc_voronoi_volumes_rule(_ptr(p), len(p), VORONOI_RULES[rule], _ptr(vol), int(nthreads))
     if over:
         raise RuntimeError("%d Voronoi cells exceeded the face/vertex limits" % over)
     return vol
 
 
-def voronoi_density(points, average=False, sx=0, cutoff=1.0, nthreads=0):
-    """estimateDensityV3D (functions.h:161-258) with the clipped volumes."""
+def voronoi_density(points, average=False, sx=0, cutoff=1.0, nthreads=0, rule="hull"):
+    """estimateDensityV3D (functions.h:161-258) with the clipped volumes of
+    the given rule."""
     p = np.ascontiguousarray(points, dtype=np.float64)
     d = np.zeros(len(p), np.float32)
-    over = lib().orc_voronoi_density(_ptr(p), len(p), int(bool(average)), int(sx), float(cutoff), _ptr(d),
-                                     int(nthreads))
+    over = lib().orc_voronoi_density_rule(_ptr(p), len(p), VORONOI_RULES[rule], int(bool(average)), int(sx),
+                                          float(cutoff), _ptr(d), int(nthreads))
     if over:
         raise RuntimeError("%d Voronoi cells exceeded the face/vertex limits" % over)
     return d

@@ -291,9 +291,16 @@ double cell_volume(const Cell<LM>& c) {
     return vol / 6.0;
 }
 
+// Hull rule: a cell that still has a face of the first box is built again
+// from a box wider than the point cloud by kWideBox times its largest extent
+// on every side, visiting every hash cell until the radius bound closes; one
+// that still keeps a box face then is unbounded (a hull point).
+constexpr double kWideBox = 65536.0;
+
 struct Hash {
     double lo[3];
     double blo[3], bhi[3];   // bounding box of the points widened by one hash cell
+    double wlo[3], whi[3];   // the same box widened by kWideBox times the largest extent
     double h;
     int n[3];
     std::vector<int> start, idx;
@@ -313,6 +320,10 @@ Hash build_hash(const double* p, long np) {
     H.h = std::cbrt(vol * 6.0 / (double)np);
     for (int a = 0; a < 3; a++) H.n[a] = (int)((hi[a] - H.lo[a]) / H.h) + 1;
     for (int a = 0; a < 3; a++) { H.blo[a] = H.lo[a] - H.h; H.bhi[a] = hi[a] + H.h; }
+    double ext = 0.0;
+    for (int a = 0; a < 3; a++) ext = std::fmax(ext, hi[a] - H.lo[a]);
+    const double wide = kWideBox * ext;
+    for (int a = 0; a < 3; a++) { H.wlo[a] = H.lo[a] - wide; H.whi[a] = hi[a] + wide; }
     const long nc = (long)H.n[0] * H.n[1] * H.n[2];
     std::vector<int> cnt(nc + 1, 0), key(np);
     for (long i = 0; i < np; i++) {
@@ -336,8 +347,12 @@ Hash build_hash(const double* p, long np) {
 // has a box face when it outgrows them is taken as unbounded (0, overflow
 // cleared) and a closed cell's overflow stays set; with the first (small)
 // limits any overflow is left set so that the cell is built again.
+// wide = false: the box of the points widened by one hash cell, at most RMAX
+// shells; wide = true (hull rule, second stage): the box widened by kWideBox
+// times the points' extent, every shell.
 template <class LM>
-double point_volume(const double* p, long i, const Hash& H, Cell<LM>& c, ClipScratch<LM>& w, bool final_pass) {
+double point_volume(const double* p, long i, const Hash& H, Cell<LM>& c, ClipScratch<LM>& w, bool final_pass,
+                    bool wide = false) {
     const V3 pi = {p[3 * i], p[3 * i + 1], p[3 * i + 2]};
     int c3[3];
     for (int a = 0; a < 3; a++) {
@@ -345,9 +360,15 @@ double point_volume(const double* p, long i, const Hash& H, Cell<LM>& c, ClipScr
         c3[a] = k < 0 ? 0 : (k >= H.n[a] ? H.n[a] - 1 : k);
     }
     double blo[3], bhi[3];
-    for (int a = 0; a < 3; a++) { blo[a] = H.blo[a] - p[3 * i + a]; bhi[a] = H.bhi[a] - p[3 * i + a]; }
+    for (int a = 0; a < 3; a++) {
+        blo[a] = (wide ? H.wlo[a] : H.blo[a]) - p[3 * i + a];
+        bhi[a] = (wide ? H.whi[a] : H.bhi[a]) - p[3 * i + a];
+    }
     init_box(c, blo, bhi);
-    for (int r = 0; r <= RMAX; r++) {
+    int rlim = RMAX;
+    if (wide)
+        for (int a = 0; a < 3; a++) rlim = H.n[a] > rlim ? H.n[a] : rlim;
+    for (int r = 0; r <= rlim; r++) {
         for (int dx = -r; dx <= r; dx++)
             for (int dy = -r; dy <= r; dy++)
                 for (int dz = -r; dz <= r; dz++) {
@@ -383,28 +404,29 @@ double point_volume(const double* p, long i, const Hash& H, Cell<LM>& c, ClipScr
 
 }  // namespace vor
 
-extern "C" {
+namespace vor {
 
-// Unnormalised cell volumes (0 on the hull) of np points (np x 3 doubles);
-// returns the number of cells that overflowed the fixed face/vertex limits.
-long orc_voronoi_volumes(const double* pts, long np, double* vol, int nthreads) {
-    vor::Hash H = vor::build_hash(pts, np);
+// Volumes of the listed points (all when list is null) with the small limits,
+// cells that outgrow them again with the large ones; returns the closed cells
+// that outgrew the large limits.
+long volumes_pass(const double* pts, long n, const long* list, const Hash& H, bool wide, double* vol, int nthreads) {
     long over = 0;
 #pragma omp parallel num_threads(nthreads > 0 ? nthreads : omp_get_max_threads()) reduction(+ : over)
     {
-        auto* cs = new vor::Cell<vor::SmallLimits>;
-        auto* ws = new vor::ClipScratch<vor::SmallLimits>;
-        vor::Cell<vor::LargeLimits>* cl = nullptr;
-        vor::ClipScratch<vor::LargeLimits>* wl = nullptr;
-#pragma omp for schedule(dynamic, 256)
-        for (long i = 0; i < np; i++) {
-            vol[i] = vor::point_volume(pts, i, H, *cs, *ws, false);
+        auto* cs = new Cell<SmallLimits>;
+        auto* ws = new ClipScratch<SmallLimits>;
+        Cell<LargeLimits>* cl = nullptr;
+        ClipScratch<LargeLimits>* wl = nullptr;
+#pragma omp for schedule(dynamic, 16)
+        for (long k = 0; k < n; k++) {
+            const long i = list ? list[k] : k;
+            vol[i] = point_volume(pts, i, H, *cs, *ws, false, wide);
             if (cs->overflow) {   // second pass with the large limits
                 if (!cl) {
-                    cl = new vor::Cell<vor::LargeLimits>;
-                    wl = new vor::ClipScratch<vor::LargeLimits>;
+                    cl = new Cell<LargeLimits>;
+                    wl = new ClipScratch<LargeLimits>;
                 }
-                vol[i] = vor::point_volume(pts, i, H, *cl, *wl, true);
+                vol[i] = point_volume(pts, i, H, *cl, *wl, true, wide);
                 over += cl->overflow;
             }
         }
@@ -414,6 +436,35 @@ long orc_voronoi_volumes(const double* pts, long np, double* vol, int nthreads) 
         delete wl;
     }
     return over;
+}
+
+}  // namespace vor
+
+extern "C" {
+
+// Unnormalised cell volumes of np points (np x 3 doubles); returns the number
+// of cells that overflowed the fixed face/vertex limits.
+//   rule 0 (hull, the reference's functions.h:148-150): 0 only for cells
+//     that are unbounded, i.e. points on the convex hull.  Cells that keep a
+//     face of the first box (the points' bounding box widened by one hash
+//     cell) are built again from the wide box; those that still keep a box
+//     face are the unbounded ones.
+//   rule 1 (clipped): 0 for every cell that keeps a face of the first box
+//     (hull points and interior points whose cell reaches beyond it).
+long orc_voronoi_volumes_rule(const double* pts, long np, int rule, double* vol, int nthreads) {
+    vor::Hash H = vor::build_hash(pts, np);
+    long over = vor::volumes_pass(pts, np, nullptr, H, false, vol, nthreads);
+    if (rule == 0) {
+        std::vector<long> open;
+        for (long i = 0; i < np; i++)
+            if (vol[i] == 0.0) open.push_back(i);
+        over += vor::volumes_pass(pts, (long)open.size(), open.data(), H, true, vol, nthreads);
+    }
+    return over;
+}
+
+long orc_voronoi_volumes(const double* pts, long np, double* vol, int nthreads) {
+    return orc_voronoi_volumes_rule(pts, np, 1, vol, nthreads);
 }
 
 // The tail of estimateDensityV3D (functions.h:212-246) on given volumes:
@@ -443,12 +494,17 @@ void orc_density_from_volumes(const double* vol, long np, int average, int sx, d
     for (long i = 0; i < np; i++) dcf[i] = (float)((double)dcf[i] * inv);
 }
 
-// estimateDensityV3D: clipped cell volumes, then the tail above.
-long orc_voronoi_density(const double* pts, long np, int average, int sx, double cutoff, float* dcf, int nthreads) {
+// estimateDensityV3D: cell volumes by the given rule, then the tail above.
+long orc_voronoi_density_rule(const double* pts, long np, int rule, int average, int sx, double cutoff, float* dcf,
+                              int nthreads) {
     std::vector<double> vol(np);
-    const long over = orc_voronoi_volumes(pts, np, vol.data(), nthreads);
+    const long over = orc_voronoi_volumes_rule(pts, np, rule, vol.data(), nthreads);
     orc_density_from_volumes(vol.data(), np, average, sx, cutoff, dcf);
     return over;
+}
+
+long orc_voronoi_density(const double* pts, long np, int average, int sx, double cutoff, float* dcf, int nthreads) {
+    return orc_voronoi_density_rule(pts, np, 1, average, sx, cutoff, dcf, nthreads);
 }
 
 }  // extern "C"

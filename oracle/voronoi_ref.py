@@ -45,8 +45,14 @@ def voronoi_volumes(points):
     return vol[inv]
 
 
-def estimate_density(points):
-    """estimateDensityV3D without averaging: float32 weights = volume / max."""
-    cell = voronoi_volumes(points).astype(np.float32)
+def normalise(vol):
+    """The tail of estimateDensityV3D without averaging (functions.h:234-245):
+    float32 cell volumes times the double reciprocal of their maximum."""
+    cell = np.asarray(vol).astype(np.float32)
     mx = float(cell.max())
     return (cell.astype(np.float64) * (1.0 / mx)).astype(np.float32)
+
+
+def estimate_density(points):
+    """estimateDensityV3D without averaging: float32 weights = volume / max."""
+    return normalise(voronoi_volumes(points))

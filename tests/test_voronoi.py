@@ -2,8 +2,10 @@
 
 CPU: the oracle's clipping statement against Qhull (scipy) restating the
 reference's Delaunay-dual sum, and invariants (regular lattice, translation
-and scale).  GPU: the device kernel against the oracle bit for bit, and the
-per-bin DCF inside CG."""
+and scale), for both rules: "hull" (the reference's: 0 only on the convex
+hull) and "clipped" (the per-bin extension: 0 also for cells reaching beyond
+the points' widened bounding box).  GPU: the device kernel against the oracle
+bit for bit, and the per-bin DCF inside CG."""
 import importlib
 
 import numpy as np
@@ -19,14 +21,22 @@ def lattice(n=8):
     return np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
 
 
-def test_lattice_interior_cells_are_unit_cubes(orc):
+@pytest.mark.parametrize("rule", ["hull", "clipped"])
+def test_lattice_interior_cells_are_unit_cubes(orc, rule):
     p = lattice(7)
-    vol = orc.voronoi_volumes(p)
+    vol = orc.voronoi_volumes(p, rule=rule)
     interior = np.all((p > 1e-9) & (p < 1 - 1e-9), axis=1)
     np.testing.assert_allclose(vol[interior], (1 / 6.0) ** 3, rtol=1e-12)
     assert np.all(vol[~interior] == 0)          # hull points: 0 (functions.h:148-150)
-    d = orc.voronoi_density(p)
+    d = orc.voronoi_density(p, rule=rule)
     assert np.all(d[interior] == 1.0) and np.all(d[~interior] == 0)
+
+
+def _agree_with_qhull(a, b, rtol):
+    """Hull rule against the Qhull dual sum: the same zero set (the points
+    on the convex hull) and the same finite volumes."""
+    np.testing.assert_array_equal(a == 0, b == 0)
+    np.testing.assert_allclose(a[b > 0], b[b > 0], rtol=rtol)
 
 
 def test_clipping_matches_qhull_dual_sum(orc):
@@ -34,12 +44,14 @@ def test_clipping_matches_qhull_dual_sum(orc):
 
     rng = np.random.default_rng(0)
     p = rng.uniform(0, 1, (2500, 3))
-    a = orc.voronoi_volumes(p)
     b = voronoi_ref.voronoi_volumes(p)
+    _agree_with_qhull(orc.voronoi_volumes(p, rule="hull"), b, 1e-9)
+    a = orc.voronoi_volumes(p, rule="clipped")
     both = (a > 0) & (b > 0)
     np.testing.assert_allclose(a[both], b[both], rtol=1e-10)
-    # every hull point (unbounded cell) is 0 here too; the extra zeros are the
-    # near-boundary cells that reach beyond the widened bounding box
+    # clipped rule: every hull point (unbounded cell) is 0 here too; the
+    # extra zeros are the near-boundary cells that reach beyond the widened
+    # bounding box
     assert np.all(a[b == 0] == 0)
     extra = (a == 0) & (b > 0)
     assert np.all(np.min(np.minimum(p, 1 - p)[extra], 1) < 0.15)
@@ -52,12 +64,37 @@ def test_wave_caipi_interior_matches_qhull(orc):
     traj = synth.wave_caipi_trajectory(grid, samp, 3, 4.0)
     sm = synth.sample_mask_from_lines(synth.caipi_mask(samp[1], samp[2], (2, 2), 1), samp[0])
     p = traj[sm]
-    a = orc.voronoi_volumes(p)
     b = voronoi_ref.voronoi_volumes(p)
+    _agree_with_qhull(orc.voronoi_volumes(p, rule="hull"), b, 1e-9)
+    a = orc.voronoi_volumes(p, rule="clipped")
     both = (a > 0) & (b > 0)
     assert both.sum() > 0.85 * len(p)
     assert np.all(a[b == 0] == 0)
     np.testing.assert_allclose(a[both], b[both], rtol=1e-10)
+
+
+@pytest.mark.parametrize("grid,samp", [((64, 64, 48), (64, 32, 24)), ((48, 48, 40), (48, 24, 20))])
+def test_full_trajectory_hull_rule_matches_qhull(orc, grid, samp):
+    """The reference's own Voronoi input (every point of a wave-CAIPI
+    trajectory, acquired or not, CGSense.cpp:171-174) with its rule: zeros
+    exactly on Qhull's unbounded (convex-hull) cells, every other volume
+    finite -- up to about 300 times the points' extent for the cells just
+    inside the hull -- and equal to Qhull's dual sum; the normalised weights
+    (divided by the largest of those near-hull cells, functions.h:234-245)
+    equal the restatement's.  The agreement is 1e-9 relative rather than
+    1e-10: the trajectory's cospherical lattice points give Qhull
+    near-duplicate circumcentres."""
+    from oracle import voronoi_ref
+
+    p = synth.wave_caipi_trajectory(grid, samp, 3, 4.0)
+    a = orc.voronoi_volumes(p, rule="hull")
+    b = voronoi_ref.voronoi_volumes(p)
+    _agree_with_qhull(a, b, 1e-9)
+    assert 0.03 < (a == 0).mean() < 0.07
+    # the cells just inside the hull are kept, not zeroed as by the clipped rule
+    kept = (a > 0) & (orc.voronoi_volumes(p, rule="clipped") == 0)
+    assert kept.mean() > 0.01 and a[kept].min() > np.median(a[a > 0])
+    np.testing.assert_allclose(orc.voronoi_density(p, rule="hull"), voronoi_ref.normalise(b), rtol=2e-7, atol=0)
 
 
 def sparse_bin_points(n_lines=None):
@@ -82,7 +119,8 @@ def test_dense_cluster_interior_matches_qhull(orc):
 
     rng = np.random.default_rng(0)
     p = np.concatenate([rng.normal(0.5, 0.05, (5000, 3)), rng.uniform(0, 1, (3000, 3))])
-    a = orc.voronoi_volumes(p)
+    _agree_with_qhull(orc.voronoi_volumes(p, rule="hull"), voronoi_ref.voronoi_volumes(p), 1e-9)
+    a = orc.voronoi_volumes(p, rule="clipped")
     b = voronoi_ref.voronoi_volumes(p)
     both = (a > 0) & (b > 0)
     np.testing.assert_allclose(a[both], b[both], rtol=1e-10)
@@ -93,35 +131,41 @@ def test_dense_cluster_interior_matches_qhull(orc):
 
 def test_sparse_bin_cells_fit_the_large_limits(orc):
     p = sparse_bin_points()
-    vol = orc.voronoi_volumes(p)          # raises if a closed cell outgrew the limits
+    vol = orc.voronoi_volumes(p, rule="clipped")   # raises if a closed cell outgrew the limits
     assert (vol > 0).mean() > 0.75
 
 
-def test_density_invariant_to_translation_and_scale(orc):
+@pytest.mark.parametrize("rule", ["hull", "clipped"])
+def test_density_invariant_to_translation_and_scale(orc, rule):
     rng = np.random.default_rng(1)
     p = rng.uniform(0, 1, (1500, 3))
-    d0 = orc.voronoi_density(p)
-    d1 = orc.voronoi_density(p * 0.5 + np.array([0.1, 0.2, 0.3]))
+    d0 = orc.voronoi_density(p, rule=rule)
+    d1 = orc.voronoi_density(p * 0.5 + np.array([0.1, 0.2, 0.3]), rule=rule)
     np.testing.assert_allclose(d1, d0, rtol=1e-9, atol=1e-12)
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["uniform", "lattice", "wave", "sparse_bin"])
-def test_device_voronoi_bit_exact(pkg, orc, case):
+@pytest.mark.parametrize("case,rule", [("uniform", "clipped"), ("lattice", "clipped"), ("wave", "clipped"),
+                                       ("sparse_bin", "clipped"), ("uniform", "hull"), ("lattice", "hull"),
+                                       ("wave", "hull"), ("full_trajectory", "hull")])
+def test_device_voronoi_bit_exact(pkg, orc, case, rule):
     if case == "uniform":
         p = np.random.default_rng(2).uniform(0, 1, (20000, 3))
     elif case == "lattice":
         p = lattice(10)
     elif case == "sparse_bin":
         p = sparse_bin_points()
+    elif case == "full_trajectory":
+        p = synth.wave_caipi_trajectory((64, 64, 48), (64, 32, 24), 3, 4.0)
     else:
         grid, samp = (128, 128, 96), (128, 64, 48)
         traj = synth.wave_caipi_trajectory(grid, samp, 3, 4.0)
         p = traj[synth.sample_mask_from_lines(synth.caipi_mask(samp[1], samp[2], (2, 2), 1), samp[0])]
-    got = pkg.cgs.voronoi_volumes(p)
-    ref = orc.voronoi_volumes(p)
+    print("device Voronoi %s, %s rule, %d points" % (case, rule, len(p)), flush=True)
+    got = pkg.cgs.voronoi_volumes(p, rule=rule)
+    ref = orc.voronoi_volumes(p, rule=rule)
     np.testing.assert_array_equal(got, ref)
-    np.testing.assert_array_equal(pkg.cgs.voronoi_density(p), orc.voronoi_density(p))
+    np.testing.assert_array_equal(pkg.cgs.voronoi_density(p, rule=rule), orc.voronoi_density(p, rule=rule))
 
 
 @pytest.mark.gpu
@@ -135,7 +179,7 @@ def test_bin_voronoi_dcf_in_cg(pkg, orc):
     x, _, _ = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=7)
     act = np.flatnonzero(P["data"][0] != 0)
     w = np.zeros(P["data"].shape[1], np.float32)
-    w[act] = orc.voronoi_density(P["traj"][act])
+    w[act] = orc.voronoi_density(P["traj"][act], rule="clipped")   # the per-bin rule
     P["oracle"].set_density(w)
     xr, _, _ = P["oracle"].cg_sense(P["data"], thresh=0.0, max_iter=7, prec=64)
     P["oracle"].set_density(None)
@@ -172,8 +216,9 @@ def test_device_density_averaging_bit_exact(pkg, orc):
     point of a wave-CAIPI trajectory (the reference's own input: all lines)."""
     grid, samp = (64, 64, 48), (64, 32, 24)
     p = synth.wave_caipi_trajectory(grid, samp, 3, 4.0)
-    vol = orc.voronoi_volumes(p)
-    for cutoff in (1.0, 0.75):
-        print("density averaging: cutoff %.2f" % cutoff, flush=True)
-        got = pkg.cgs.voronoi_density(p, average=True, sx=samp[0], cutoff=cutoff)
-        np.testing.assert_array_equal(got, orc.density_from_volumes(vol, True, samp[0], cutoff))
+    for rule in ("hull", "clipped"):
+        vol = orc.voronoi_volumes(p, rule=rule)
+        for cutoff in (1.0, 0.75):
+            print("density averaging: %s rule, cutoff %.2f" % (rule, cutoff), flush=True)
+            got = pkg.cgs.voronoi_density(p, average=True, sx=samp[0], cutoff=cutoff, rule=rule)
+            np.testing.assert_array_equal(got, orc.density_from_volumes(vol, True, samp[0], cutoff))
