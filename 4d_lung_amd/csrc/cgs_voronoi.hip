@@ -8,9 +8,13 @@
 // by one hash cell and intersect it with the bisector half-spaces of the other
 // points, visiting a uniform hash grid shell by shell in a fixed order until
 // no unvisited point can cut the cell (the farthest vertex is closer than half
-// the distance of the next shell).  Cells that keep a face of the box are
-// taken as unbounded (volume 0): the hull points, and interior points whose
-// Voronoi region reaches more than one hash cell beyond the point cloud.  The
+// the distance of the next shell).  Clipped rule (per-bin weights): cells
+// that keep a face of the box get volume 0 -- the hull points, and interior
+// points whose Voronoi region reaches more than one hash cell beyond the
+// point cloud.  Hull rule (the reference's): those cells are built again
+// from a box wider than the point cloud by kVorWideBox times its extent,
+// visiting every hash cell until the radius bound closes, and only the cells
+// that still keep a box face (the unbounded ones) get 0.  The
 // volume is the reference's face sum area * |q - p| / 6 with areas as
 // triangle fans from the first vertex.  A cell is a pool of vertices and
 // faces as rings of pool indices (about 12.7 KB with the first-pass limits,

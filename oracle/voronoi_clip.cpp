@@ -15,10 +15,13 @@
 // the farthest cell vertex is within half the distance that any unvisited
 // point can have.  A cell that still has a face of the box is taken as
 // unbounded and gets volume 0, as points on the convex hull do in
-// volume_of_finite_dual_cell (functions.h:148-150); this also zeroes the
-// cells of interior points whose Voronoi region reaches more than one hash
-// cell beyond the point cloud (near-degenerate boundary cells whose CGAL
-// volume exceeds the whole k-space).  Volume = sum over faces of area *
+// volume_of_finite_dual_cell (functions.h:148-150).  Clipped rule: this also
+// zeroes the cells of interior points whose Voronoi region reaches more than
+// one hash cell beyond the point cloud (near-degenerate boundary cells whose
+// CGAL volume exceeds the whole k-space).  Hull rule (the reference's): such
+// cells are built again from a box kWideBox times wider than the point cloud
+// over every hash cell, and only the cells still keeping a box face get 0.
+// Volume = sum over faces of area *
 // |q - p| / 6 (functions.h:156-158), face areas as triangle fans from the
 // first vertex (functions.h:117-124).  Cells are built with small fixed
 // limits (faces, vertices per face, distinct vertices) and, when they outgrow
