@@ -75,6 +75,7 @@ def lib():
         "cgs_set_num_bins": (I, [P, I]),
         "cgs_set_bin_group": (I, [P, I]),
         "cgs_set_pruning": (I, [P, I]),
+        "cgs_set_grid_lists": (I, [P, I]),
         "cgs_get_pruning": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_get_prune_fractions": (I, [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "cgs_get_bin_group": (I, [P, ctypes.POINTER(ctypes.c_int)]),
@@ -280,6 +281,11 @@ class Context:
     def set_pruning(self, mode):
         """Column pruning of the bins' x passes: 0 off, 1 automatic, 2 on."""
         self._ck(self._L.cgs_set_pruning(self.h, int(mode)))
+
+    def set_grid_lists(self, on):
+        """Per-tile gridding candidate lists built when the bins are packed
+        (True, the default) or found by a scan in every gridding launch."""
+        self._ck(self._L.cgs_set_grid_lists(self.h, int(bool(on))))
 
     def pruning(self):
         """Whether the packed bins use column pruning."""

@@ -91,6 +91,11 @@ struct cgs_ctx {
     int* d_glist = nullptr;        // [nbins][pr_gstride]
     int* d_gcnt = nullptr;         // [nbins]
     uint8_t* d_tneed = nullptr;    // [nbins][nty*ntz]
+    // per-tile gridding candidate lists of the packed bins (built with them)
+    int grid_lists = 1;            // 1: build and use them; 0: gridding scans every launch
+    int* d_tlist = nullptr;
+    long long tlist_n = 0;
+    long long* d_tlist_off = nullptr;   // [nbins * ntiles + 1]
     std::vector<int> h_gcnt;
     std::vector<double> pr_gfrac, pr_tfrac;   // per bin: groups transformed, tile columns written
 
@@ -379,6 +384,9 @@ void free_packed(cgs_ctx* h) {
     dfree(h, h->d_glist, h->nbins * h->pr_gstride);
     dfree(h, h->d_gcnt, h->nbins);
     dfree(h, h->d_tneed, (long long)h->nbins * h->tiles.nty * h->tiles.ntz);
+    dfree(h, h->d_tlist, std::max(1LL, h->tlist_n));
+    dfree(h, h->d_tlist_off, (long long)h->nbins * h->tiles.ntiles + 1);
+    h->tlist_n = 0;
     h->prune_on = false;
     h->pr_ngrp = h->pr_gstride = 0;
     h->h_gcnt.clear();
@@ -461,6 +469,36 @@ int build_prune(cgs_ctx* h) {
     CK(hipMemcpy(h->d_glist, glist.data(), sizeof(int) * glist.size(), hipMemcpyHostToDevice));
     CK(hipMemcpy(h->d_gcnt, h->h_gcnt.data(), sizeof(int) * h->nbins, hipMemcpyHostToDevice));
     CK(hipMemcpy(h->d_tneed, tneed.data(), tneed.size(), hipMemcpyHostToDevice));
+    return CGS_OK;
+}
+
+// Gridding candidate lists of every tile of the packed bins: the samples
+// whose KB window meets the tile, in the order gridding's own scan of the 27
+// neighbouring tiles finds them.  The trajectory is fixed across CG
+// iterations, so the scan runs once here instead of in every launch.
+GridArgs grid_args(cgs_ctx* h, int b0, int nb);
+int build_grid_lists(cgs_ctx* h) {
+    const long long nt = (long long)h->nbins * h->tiles.ntiles;
+    GridArgs g = grid_args(h, 0, h->nbins);
+    g.tlist = nullptr;
+    g.tlist_off = nullptr;
+    int* cnt = nullptr;
+    CKS(dalloc(h, &cnt, nt));
+    CK(launch_grid_lists(h->kb.kw, h->tcfg, g, cnt, h->st));
+    std::vector<int> hc(nt);
+    CK(hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nt, hipMemcpyDeviceToHost, h->st));
+    CK(hipStreamSynchronize(h->st));
+    dfree(h, cnt, nt);
+    std::vector<long long> off(nt + 1, 0);
+    for (long long t = 0; t < nt; ++t) off[t + 1] = off[t] + hc[t];
+    h->tlist_n = off[nt];
+    CKS(dalloc(h, &h->d_tlist_off, nt + 1));
+    CKS(dalloc(h, &h->d_tlist, std::max(1LL, h->tlist_n)));
+    CK(hipMemcpy(h->d_tlist_off, off.data(), sizeof(long long) * (nt + 1), hipMemcpyHostToDevice));
+    g.tlist = h->d_tlist;
+    g.tlist_off = h->d_tlist_off;
+    CK(launch_grid_lists(h->kb.kw, h->tcfg, g, nullptr, h->st));
+    CK(hipStreamSynchronize(h->st));
     return CGS_OK;
 }
 
@@ -594,6 +632,7 @@ int pack_bins(cgs_ctx* h) {
     CKS(dalloc(h, &h->p_yw, std::max(1LL, mt) * h->C));
     CK(hipStreamSynchronize(h->st));
     CKS(build_prune(h));
+    if (h->grid_lists) CKS(build_grid_lists(h));
     h->packed = true;
     h->cg_ready = false;
     return CGS_OK;
@@ -729,6 +768,8 @@ GridArgs grid_args(cgs_ctx* h, int b0, int nb) {
     g.nb = nb;
     g.done = nullptr;
     g.tneed = h->prune_on ? h->d_tneed : nullptr;
+    g.tlist = h->d_tlist;
+    g.tlist_off = h->d_tlist_off;
     return g;
 }
 
@@ -1105,6 +1146,14 @@ int cgs_get_prune_fractions(const cgs_ctx* h, int bin, double* groups, double* t
     if (!h->packed || (int)h->pr_gfrac.size() != h->nbins) return CGS_ERR_STATE;
     *groups = h->pr_gfrac[bin];
     *tiles = h->pr_tfrac[bin];
+    return CGS_OK;
+}
+
+int cgs_set_grid_lists(cgs_ctx* h, int on) {
+    if (!h || on < 0 || on > 1) return CGS_ERR_ARG;
+    if (on == h->grid_lists) return CGS_OK;
+    h->grid_lists = on;
+    free_packed(h);
     return CGS_OK;
 }
 

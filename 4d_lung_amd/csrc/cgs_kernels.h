@@ -121,10 +121,19 @@ struct GridArgs {
     const int* done;       // [nbins_total] flags, or null
     const uint8_t* tneed;  // gridding: [nbins_total][nty*ntz] tile columns to write (others are
                            // never read: their x-pass groups are zero-filled), or null = all
+    // gridding: per tile the samples whose window meets it, in the order the
+    // kernel's own scan of the 27 neighbouring tiles finds them (built once
+    // per packed bin set by launch_grid_lists), or null = scan every launch
+    const int* tlist;      // sample indices (global)
+    const long long* tlist_off;   // [nbins_total * ntiles + 1]
 };
 
 hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s);
 hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s);
+// Candidate lists of every tile of bins b0..b0+nb-1 (a.b0, a.nb): with
+// counts != null the number per tile goes to counts[bin * ntiles + tile];
+// otherwise the lists are written to a.tlist at a.tlist_off.
+hipError_t launch_grid_lists(int kw, int tcfg, const GridArgs& a, int* counts, hipStream_t s);
 
 // Coordinate gradient of the sample values: for every sample i (sorted
 // position s, original index base[s].w)

@@ -103,8 +103,73 @@ __global__ __launch_bounds__(CGS_THREADS) void k_degrid(GridArgs a) {
 }
 
 // -------------------------------------------------------------- gridding
+// Candidate samples of tile (itx, ity, itz) of one bin: the samples of the
+// 3 x 3 tile columns around it (z tiles itz-1 .. itz+1) -- nine CSR ranges,
+// walked as one flat sequence in (dx, dy, sample) order -- whose window meets
+// the tile.  The block compacts them 256 at a time with a ballot and prefix
+// (so the order is fixed) into the LDS array `list` and calls
+// chunk(total) after each compaction, all threads together.
+template <int KW, int TX, int TY, int TZ, class Chunk>
+__device__ __forceinline__ void tile_candidates(const GridArgs& a, int bin, int itx, int ity, int itz, int* list,
+                                                int* wcount, int* rbeg, int* rlen, Chunk chunk) {
+    constexpr int NWAVE = CGS_THREADS / 64;
+    const int ntiles = a.t.ntiles;
+    const int* __restrict__ off = a.tile_off + (long long)bin * (ntiles + 1);
+    const int x0 = itx * TX, y0 = ity * TY, z0 = itz * TZ;
+    const int tid = threadIdx.x;
+    const int lane = tid % 64, wid = tid / 64;
+    const int tz_lo = itz > 0 ? itz - 1 : 0;
+    const int tz_hi = itz + 1 < a.t.ntz ? itz + 1 : a.t.ntz - 1;
+    if (tid < 9) {
+        const int jx = itx + tid / 3 - 1, jy = ity + tid % 3 - 1;
+        int rb = 0, rl = 0;
+        if (jx >= 0 && jx < a.t.ntx && jy >= 0 && jy < a.t.nty) {
+            const int tcol = (jx * a.t.nty + jy) * a.t.ntz;
+            rb = off[tcol + tz_lo];
+            rl = off[tcol + tz_hi + 1] - rb;
+        }
+        rbeg[tid] = rb;
+        rlen[tid] = rl;
+    }
+    __syncthreads();
+    int rpre[10];
+    rpre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) rpre[r + 1] = rpre[r] + rlen[r];
+    const int nflat = rpre[9];
+    for (int f0 = 0; f0 < nflat; f0 += CGS_THREADS) {
+        const int f = f0 + tid;
+        bool hit = false;
+        int s = 0;
+        if (f < nflat) {
+            int r = 0;
+#pragma unroll
+            for (int q = 1; q < 9; ++q) r += f >= rpre[q] ? 1 : 0;
+            s = rbeg[r] + (f - rpre[r]);
+            const int4 bs = a.s.base[s];
+            hit = bs.x < x0 + TX && bs.x + KW > x0 && bs.y < y0 + TY && bs.y + KW > y0 &&
+                  bs.z < z0 + TZ && bs.z + KW > z0;
+        }
+        const unsigned long long bal = __ballot(hit);
+        const int wpre = __popcll(bal & ((1ULL << lane) - 1ULL));
+        if (lane == 0) wcount[wid] = __popcll(bal);
+        __syncthreads();
+        int wbase = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < NWAVE; ++w) {
+            if (w < wid) wbase += wcount[w];
+            total += wcount[w];
+        }
+        if (hit) list[wbase + wpre] = s;
+        __syncthreads();
+        chunk(total);
+    }
+}
+
 // A thread owns CG coils of one column (x, y) of the tile and its TZ cells
-// along z: each tap weight is formed once per CG coils.
+// along z: each tap weight is formed once per CG coils.  The candidates come
+// from the tile's precomputed list (a.tlist) or, without one, from the scan
+// of tile_candidates; either way in the same order, so the sums are the same.
 template <int KW, int TX, int TY, int TZ, int CP, int CG>
 __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     constexpr int VS = val_stride(KW);
@@ -132,7 +197,6 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     if (a.tneed && !a.tneed[(long long)bin * (a.t.nty * a.t.ntz) + ity * a.t.ntz + itz]) return;
     const int x0 = itx * TX, y0 = ity * TY, z0 = itz * TZ;
     const int C = a.g.C;
-    const int* __restrict__ off = a.tile_off + (long long)bin * (ntiles + 1);
 
     float2 acc[NPT][CG][TZ];
 #pragma unroll
@@ -156,65 +220,19 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
         }
     }
     const int gxlo = x0 + wxlo, gxhi = x0 + wxhi;
-    const int tz_lo = itz > 0 ? itz - 1 : 0;
-    const int tz_hi = itz + 1 < a.t.ntz ? itz + 1 : a.t.ntz - 1;
 
-    // The candidates are the samples of the 3 x 3 tile columns around this one
-    // (z tiles itz-1 .. itz+1): nine CSR ranges, walked as one flat sequence
-    // in (dx, dy, sample) order.
-    if (tid < 9) {
-        const int jx = itx + tid / 3 - 1, jy = ity + tid % 3 - 1;
-        int rb = 0, rl = 0;
-        if (jx >= 0 && jx < a.t.ntx && jy >= 0 && jy < a.t.nty) {
-            const int tcol = (jx * a.t.nty + jy) * a.t.ntz;
-            rb = off[tcol + tz_lo];
-            rl = off[tcol + tz_hi + 1] - rb;
-        }
-        rbeg[tid] = rb;
-        rlen[tid] = rl;
-    }
-    __syncthreads();
-    int rpre[10];
-    rpre[0] = 0;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) rpre[r + 1] = rpre[r] + rlen[r];
-    const int nflat = rpre[9];
-    for (int f0 = 0; f0 < nflat; f0 += CGS_THREADS) {
-        // --- compact the candidates whose window meets this tile (fixed order)
-        const int f = f0 + tid;
-        bool hit = false;
-        int s = 0;
-        if (f < nflat) {
-            int r = 0;
-#pragma unroll
-            for (int q = 1; q < 9; ++q) r += f >= rpre[q] ? 1 : 0;
-            s = rbeg[r] + (f - rpre[r]);
-            const int4 bs = a.s.base[s];
-            hit = bs.x < x0 + TX && bs.x + KW > x0 && bs.y < y0 + TY && bs.y + KW > y0 &&
-                  bs.z < z0 + TZ && bs.z + KW > z0;
-        }
-        const unsigned long long bal = __ballot(hit);
-        const int wpre = __popcll(bal & ((1ULL << lane) - 1ULL));
-        if (lane == 0) wcount[wid] = __popcll(bal);
-        __syncthreads();
-        int wbase = 0, total = 0;
-#pragma unroll
-        for (int w = 0; w < NWAVE; ++w) {
-            if (w < wid) wbase += wcount[w];
-            total += wcount[w];
-        }
-        if (hit) list[wbase + wpre] = s;
-        __syncthreads();
+    // stage and accumulate `total` candidates, the j-th being entry(j)
+    auto accumulate = [&](int total, auto entry) {
         for (int j0 = 0; j0 < total; j0 += SUB) {
             const int nsub = min(SUB, total - j0);
             // --- stage window origins, KB factors and (weighted) values in LDS
             for (int t = tid; t < nsub * VS; t += CGS_THREADS) {
                 const int j = t / VS, o = t % VS;
-                svals[t] = a.s.vals[(long long)list[j0 + j] * VS + o];
+                svals[t] = a.s.vals[(long long)entry(j0 + j) * VS + o];
             }
             for (int t = tid; t < nsub * CP; t += CGS_THREADS) {
                 const int j = t / CP, c = t % CP;
-                const int sj = list[j0 + j];
+                const int sj = entry(j0 + j);
                 float2 yv = make_float2(0.0f, 0.0f);
                 if (c < C) {
                     yv = a.Y[(long long)sj * C + c];
@@ -222,7 +240,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
                 }
                 sy[t] = yv;
             }
-            if (tid < nsub) sbase[tid] = a.s.base[list[j0 + tid]];
+            if (tid < nsub) sbase[tid] = a.s.base[entry(j0 + tid)];
             __syncthreads();
             // --- accumulate, walking only the candidates whose window covers
             // one of the wave's rows (the others add nothing to its cells),
@@ -268,6 +286,22 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
             }
             __syncthreads();
         }
+    };
+    if (a.tlist) {
+        // the tile's list, 256 entries at a time through LDS (one coalesced load)
+        const long long g0 = a.tlist_off[(long long)bin * ntiles + tile];
+        const int total = (int)(a.tlist_off[(long long)bin * ntiles + tile + 1] - g0);
+        const int* __restrict__ tl = a.tlist + g0;
+        for (int c0 = 0; c0 < total; c0 += CGS_THREADS) {
+            const int n = min(CGS_THREADS, total - c0);
+            if (tid < n) list[tid] = tl[c0 + tid];
+            __syncthreads();
+            accumulate(n, [&](int j) { return list[j]; });   // ends with a barrier (n > 0)
+        }
+    } else {
+        tile_candidates<KW, TX, TY, TZ>(a, bin, itx, ity, itz, list, wcount, rbeg, rlen, [&](int total) {
+            accumulate(total, [&](int j) { return list[j]; });
+        });
     }
     // --- write the tile (every cell of the grid belongs to exactly one tile)
     float2* __restrict__ G = a.G + (long long)lbin * a.g.bp;
@@ -289,6 +323,37 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
             }
         }
     }
+}
+
+// The candidate lists of the tiles of bins b0.. (one block per tile, the
+// scan of k_grid): counts per tile (counts != null), else the lists.  Tiles
+// that gridding skips (tneed) get none.
+template <int KW, int TX, int TY, int TZ>
+__global__ __launch_bounds__(CGS_THREADS) void k_grid_lists(GridArgs a, int* counts) {
+    constexpr int NWAVE = CGS_THREADS / 64;
+    __shared__ int list[CGS_THREADS];
+    __shared__ int wcount[NWAVE];
+    __shared__ int rbeg[9], rlen[9];
+    const int ntiles = a.t.ntiles;
+    const int bin = a.b0 + (int)(blockIdx.x / ntiles);
+    const int tile = blockIdx.x % ntiles;
+    const int itz = tile % a.t.ntz;
+    const int ity = (tile / a.t.ntz) % a.t.nty;
+    const int itx = tile / (a.t.ntz * a.t.nty);
+    const long long gt = (long long)bin * ntiles + tile;
+    if (a.tneed && !a.tneed[(long long)bin * (a.t.nty * a.t.ntz) + ity * a.t.ntz + itz]) {
+        if (counts && threadIdx.x == 0) counts[gt] = 0;
+        return;
+    }
+    long long pos = counts ? 0 : a.tlist_off[gt];
+    int n = 0;
+    tile_candidates<KW, TX, TY, TZ>(a, bin, itx, ity, itz, list, wcount, rbeg, rlen, [&](int total) {
+        if (!counts)
+            for (int t = threadIdx.x; t < total; t += CGS_THREADS) ((int*)a.tlist)[pos + n + t] = list[t];
+        n += total;
+        __syncthreads();   // list is rewritten by the next compaction
+    });
+    if (counts && threadIdx.x == 0) counts[gt] = n;
 }
 
 // ------------------------------------------------------ coordinate gradient
@@ -428,6 +493,33 @@ hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s) {
         case 3: return grid_tile<3>(tcfg, a, s);
         case 5: return grid_tile<5>(tcfg, a, s);
         case 7: return grid_tile<7>(tcfg, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int KW, int TX, int TY, int TZ>
+static hipError_t lists_tile(const GridArgs& a, int* counts, hipStream_t s) {
+    const dim3 grid((unsigned)((long long)a.nb * a.t.ntiles));
+    k_grid_lists<KW, TX, TY, TZ><<<grid, CGS_THREADS, 0, s>>>(a, counts);
+    return hipGetLastError();
+}
+
+template <int KW>
+static hipError_t lists_kw(int tcfg, const GridArgs& a, int* counts, hipStream_t s) {
+    if (tcfg < 0 || tcfg >= grid_tile_configs()) return hipErrorInvalidValue;
+    const int tx = kTileCfg[tcfg][0], ty = kTileCfg[tcfg][1], tz = kTileCfg[tcfg][2];
+    if (tx == 8 && ty == 8 && tz == 8) return lists_tile<KW, 8, 8, 8>(a, counts, s);
+    if (tx == 8 && ty == 8 && tz == 4) return lists_tile<KW, 8, 8, 4>(a, counts, s);
+    if (tx == 4 && ty == 8 && tz == 8) return lists_tile<KW, 4, 8, 8>(a, counts, s);
+    if (tx == 4 && ty == 4 && tz == 8) return lists_tile<KW, 4, 4, 8>(a, counts, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_grid_lists(int kw, int tcfg, const GridArgs& a, int* counts, hipStream_t s) {
+    switch (kw) {
+        case 3: return lists_kw<3>(tcfg, a, counts, s);
+        case 5: return lists_kw<5>(tcfg, a, counts, s);
+        case 7: return lists_kw<7>(tcfg, a, counts, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -1,6 +1,7 @@
 """Summaries of rocprofv3 runs of bench.py (kernel trace and HBM counters).
 
     python 4d_lung_amd/profiling.py TRACE_DIR FETCH_DIR WRITE_DIR WORKLOAD OUT_DIR [BINS_PER_LAUNCH]
+    python 4d_lung_amd/profiling.py raw RD_DIR WR_DIR TRACE_DIR WORKLOAD OUT_DIR [BINS_PER_LAUNCH]
 
 TRACE_DIR holds ``run_kernel_trace.csv`` of ``rocprofv3 --kernel-trace
 --stats -f csv -o run``; FETCH_DIR / WRITE_DIR hold ``run_counter_collection.csv``
@@ -86,7 +87,99 @@ def counter_bytes(pmc_dir, counter):
     return per, max(copies), scale
 
 
+# best streaming copy measured on the box (read + write, 4 GiB buffers)
+COPY_GBS = 5605.0
+
+RAW_READ = ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_BUBBLE_sum", "TCC_EA0_RDREQ_DRAM_sum")
+RAW_WRITE = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum", "TCC_EA0_WRREQ_DRAM_sum")
+
+
+def raw_counters(pmc_dir, names):
+    """Per stage and for the 1 GiB calibration copy (its largest dispatch):
+    {stage: {counter: median per dispatch}}."""
+    disp = collections.OrderedDict()
+    for r in _rows(os.path.join(pmc_dir, "run_counter_collection.csv")):
+        d = disp.setdefault(r["Dispatch_Id"], {"name": r["Kernel_Name"]})
+        d[r["Counter_Name"]] = float(r["Counter_Value"])
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    copy = collections.defaultdict(float)
+    cnt = {-1: 0, 1: 0}
+    for d in disp.values():
+        if "copyBuffer" in d["name"]:
+            for c in names:
+                copy[c] = max(copy[c], d.get(c, 0.0))
+            continue
+        st = stage_of(d["name"], cnt)
+        if st:
+            for c in names:
+                per[st][c].append(d.get(c, 0.0))
+    out = {st: {c: statistics.median(v) for c, v in cs.items()} for st, cs in per.items()}
+    out["calibration copy"] = dict(copy)
+    return out
+
+
+def raw_main(argv):
+    """raw RD_DIR WR_DIR TRACE_DIR WORKLOAD OUT_DIR [BINS_PER_LAUNCH]: table of
+    the TCC request counters per stage (profiles/counters_WORKLOAD_requests.md)."""
+    rd_dir, wr_dir, trace_dir, workload, out_dir = argv[2:7]
+    nb = int(argv[7]) if len(argv) > 7 else 1
+    rd = raw_counters(rd_dir, RAW_READ)
+    wr = raw_counters(wr_dir, RAW_WRITE)
+    dur, _ = kernel_durations(trace_dir)
+    med = statistics.median
+    path = os.path.join(out_dir, "counters_%s_requests.md" % workload)
+    with open(path, "w") as f:
+        f.write("# TCC request counters of the CG-iteration kernels, workload %s\n\n" % workload)
+        f.write("Commands (on the GPU box, from `/tmp` with `TMPDIR=/tmp`, `$R` = repository root, `$A` = the\n"
+                "bench.py arguments of the kernel trace beside this file):\n\n```\n"
+                "rocprofv3 --pmc %s -f csv -d $R/gpurun_out/pmc_rawrd_%s -o run -- python3 $R/bench.py $A\n"
+                "rocprofv3 --pmc %s -f csv -d $R/gpurun_out/pmc_rawwr_%s -o run -- python3 $R/bench.py $A\n"
+                "python3 4d_lung_amd/profiling.py raw gpurun_out/pmc_rawrd_%s gpurun_out/pmc_rawwr_%s "
+                "gpurun_out/ktrace_%s %s profiles%s\n```\n\n"
+                % (" ".join(RAW_READ), workload, " ".join(RAW_WRITE), workload, workload, workload, workload,
+                   workload, " %d" % nb if nb > 1 else ""))
+        f.write("Median per dispatch%s; the calibration copy moves 1 GiB each way.  Read bytes are\n"
+                "given for 64-byte and for 128-byte requests (the counters cannot tell them apart here,\n"
+                "see below); writes are 64-byte requests.\n\n" % (" (launches of %d bins)" % nb if nb > 1 else ""))
+        f.write("| stage | RDREQ | RDREQ_32B | BUBBLE | RDREQ_DRAM | read GB (64 B / 128 B) | WRREQ | WRREQ_64B |"
+                " WRREQ_DRAM | write GB | median us | GB/s at 128 B reads |\n")
+        f.write("|---|---|---|---|---|---|---|---|---|---|---|---|\n")
+        for st in ["calibration copy"] + sorted(dur, key=lambda k: -med(dur[k])):
+            if st not in rd or st not in wr:
+                continue
+            r, w = rd[st], wr[st]
+            r64 = r["TCC_EA0_RDREQ_sum"] * 64 / 1e9
+            wgb = (w["TCC_EA0_WRREQ_64B_sum"] * 64 + (w["TCC_EA0_WRREQ_sum"] - w["TCC_EA0_WRREQ_64B_sum"]) * 32) / 1e9
+            us = med(dur[st]) if st in dur else None
+            rate = "%.0f" % ((2 * r64 + wgb) / (us * 1e-6)) if us else ""
+            f.write("| %s | %.4g | %.4g | %.4g | %.4g | %.3f / %.3f | %.4g | %.4g | %.4g | %.3f | %s | %s |\n" % (
+                st, r["TCC_EA0_RDREQ_sum"], r["TCC_EA0_RDREQ_32B_sum"], r["TCC_BUBBLE_sum"],
+                r["TCC_EA0_RDREQ_DRAM_sum"], r64, 2 * r64, w["TCC_EA0_WRREQ_sum"], w["TCC_EA0_WRREQ_64B_sum"],
+                w["TCC_EA0_WRREQ_DRAM_sum"], wgb, "%.1f" % us if us else "", rate))
+        f.write(RAW_NOTE)
+    return 0
+
+
+RAW_NOTE = """
+The 1 GiB calibration copy issues 8.39 M read requests, 128 bytes each, yet
+TCC_BUBBLE (128-byte read requests) and RDREQ_32B read 0, so FETCH_SIZE
+(128 BUBBLE + 64 (RDREQ - BUBBLE - RDREQ_32B) + 32 RDREQ_32B bytes) counts
+them as 64 bytes: 0.5 GiB.  Writes are counted exactly (WRREQ_64B x 64 bytes =
+1 GiB).  A read request of these counters is therefore 64 or 128 bytes, and
+the counters do not say which.  The strided FFT passes issue half as many read
+as write requests, i.e. 128-byte reads, and at 128 bytes their reads equal
+their algorithmic bytes.  RDREQ_DRAM equals RDREQ everywhere: hits in the
+memory-side cache are not told apart from HBM reads.  The traffic summaries
+(FETCH_SIZE x 2 + WRITE_SIZE) are therefore the L2-to-memory bytes at
+128-byte reads: exact for kernels that read whole 128-byte lines, an upper
+bound (up to 2x on reads) for kernels that issue 64-byte reads, and inclusive
+of memory-side-cache hits.
+"""
+
+
 def main(argv):
+    if len(argv) > 1 and argv[1] == "raw":
+        return raw_main(argv)
     trace_dir, fetch_dir, write_dir, workload, out_dir = argv[1:6]
     nb = int(argv[6]) if len(argv) > 6 else 1
     dur, names = kernel_durations(trace_dir)
@@ -104,12 +197,18 @@ def main(argv):
         f.write("# FETCH_SIZE / WRITE_SIZE (KiB per dispatch), median over the run's launches,\n")
         f.write("# scaled by the 1 GiB calibration copy: it read %.0f KiB (x%.4f) and wrote %.0f KiB (x%.4f)\n"
                 % (fcal, fs, wcal, ws))
-        f.write("# stage fetch_bytes write_bytes\n")
+        f.write("# (the copy's reads are 128-byte requests that FETCH_SIZE counts as 64 bytes: the read\n"
+                "# figures assume the same of every kernel and are an upper bound where a kernel issues\n"
+                "# 64-byte read requests; they include what the memory-side cache serves)\n")
+        f.write("# stage fetch_bytes write_bytes implied_GBps (launch bytes / median kernel-trace time)\n")
         for st in order:
             if st in fetch and st in write:
                 fb = med(fetch[st]) / nb
                 wb = med(write[st]) / nb
-                f.write("%s %.6g %.6g\n" % (st, fb, wb))
+                gbs = (fb + wb) * nb / (med(dur[st]) * 1e-6) / 1e9
+                f.write("%s %.6g %.6g %.0f%s\n" % (st, fb, wb, gbs,
+                                                    "  # above the best copy rate (%.0f GB/s): 64-byte reads" % COPY_GBS
+                                                    if gbs > COPY_GBS else ""))
     kpath = os.path.join(out_dir, "kernels_%s.md" % workload)
     head = ""
     if os.path.exists(kpath):   # the description above the tables is kept
@@ -145,7 +244,7 @@ def read_traffic(path):
                 per_bin = True
             if line.startswith("#") or not line.strip():
                 continue
-            st, fb, wb = line.split()
+            st, fb, wb = line.split("#")[0].split()[:3]
             out[st] = (float(fb), float(wb))
     return out, per_bin
 

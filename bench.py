@@ -147,6 +147,7 @@ def bench_binned(args, cfg, pkg, torch):
     ctx, traj, maps, bins, voronoi_s = wl["ctx"], wl["traj"], wl["maps"], wl["bins"], wl["voronoi_seconds"]
     voronoi = voronoi_s is not None
     counts = [int(bb[0].size) for bb in bins]
+    ctx.set_grid_lists(args.grid_lists)
     ctx.set_num_bins(B)
 
     def load_bins():
@@ -348,6 +349,9 @@ def main():
     ap.add_argument("--pruning", type=int, default=-1, choices=(-1, 0, 1, 2),
                     help="column pruning of the x passes: 0 off, 1 automatic, 2 on (-1: the library's default)")
     ap.add_argument("--no-plain", action="store_true", help="binned workloads: skip timing the plain path")
+    ap.add_argument("--grid-lists", type=int, default=1, choices=(0, 1),
+                    help="per-tile gridding candidate lists built at packing (1, the library's default) or a scan "
+                         "in every gridding launch (0)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (the reference kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")
@@ -395,6 +399,7 @@ def main():
     ctx = pkg.Context(grid, samp, C)
     if args.pruning >= 0:
         ctx.set_pruning(args.pruning)
+    ctx.set_grid_lists(args.grid_lists)
     ctx.set_trajectory(traj)
     ctx.calculate_kernel_ft()
     ctx.set_maps(maps)

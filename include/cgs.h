@@ -120,6 +120,13 @@ int cgs_get_pruning(const cgs_ctx* h, int* active);
  * gridding tile columns that are written (1 and 1 without pruning); valid
  * while the bins stay packed (after cgs_cg_prepare). */
 int cgs_get_prune_fractions(const cgs_ctx* h, int bin, double* groups, double* tiles);
+/* Per-tile gridding candidate lists (on = 1, the default): the samples whose
+ * KB window meets each gridding tile are listed once when the bins are packed
+ * (the trajectory is fixed across CG iterations) instead of being found by a
+ * scan of the 27 neighbouring tiles in every gridding launch (on = 0).  Same
+ * candidates in the same order, so the same images bit for bit.  Takes effect
+ * when the bins are next packed.  No reference counterpart. */
+int cgs_set_grid_lists(cgs_ctx* h, int on);
 /* Compact form: m acquired samples with original indices idx[m] (any order),
  * values y[c][m] complex and optional weights dcf[m]. */
 int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, const float* y, const float* dcf);
