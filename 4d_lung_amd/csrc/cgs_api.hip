@@ -856,7 +856,7 @@ int cgs_create(const int grid[3], const int samp[3], int ncoil, cgs_ctx** out) {
     *out = nullptr;
     for (int a = 0; a < 3; ++a)
         if (grid[a] < 2 || samp[a] < 1) return CGS_ERR_ARG;
-    if (ncoil < 1 || ncoil > 32) return CGS_ERR_ARG;
+    if (ncoil < 1 || ncoil > CGS_MAX_COILS) return CGS_ERR_ARG;
     for (int a = 0; a < 3; ++a)
         if (!fft_len_supported(grid[a])) return CGS_ERR_ARG;
     cgs_ctx* h = new cgs_ctx();

@@ -126,6 +126,7 @@ struct GridArgs {
     // per packed bin set by launch_grid_lists), or null = scan every launch
     const int* tlist;      // sample indices (global)
     const long long* tlist_off;   // [nbins_total * ntiles + 1]
+    int c0;                // gridding: first coil of this launch (coils in groups of at most 32)
 };
 
 hipError_t launch_degrid(int kw, const GridArgs& a, hipStream_t s);

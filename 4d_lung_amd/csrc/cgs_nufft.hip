@@ -234,8 +234,8 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
                 const int j = t / CP, c = t % CP;
                 const int sj = entry(j0 + j);
                 float2 yv = make_float2(0.0f, 0.0f);
-                if (c < C) {
-                    yv = a.Y[(long long)sj * C + c];
+                if (a.c0 + c < C) {
+                    yv = a.Y[(long long)sj * C + a.c0 + c];
                     if (a.gdcf) yv = cscale(yv, a.gdcf[sj]);
                 }
                 sy[t] = yv;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
                 for (int k = 0; k < NPT; ++k) {
                     const int e = tid + k * CGS_THREADS;
                     const int g = e % NG, col = e / NG;
-                    if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && g * CG < C) {
+                    if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && a.c0 + g * CG < C) {
                         const int ox = x0 + col / TY - bs.x;
                         const int oy = y0 + col % TY - bs.y;
                         if (ox >= 0 && ox < KW && oy >= 0 && oy < KW) {
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
     for (int k = 0; k < NPT; ++k) {
         const int e = tid + k * CGS_THREADS;
         const int g = e % NG, col = e / NG;
-        if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && g * CG < C) {
+        if ((NPAIR % CGS_THREADS == 0 || e < NPAIR) && a.c0 + g * CG < C) {
             const int x = x0 + col / TY, y = y0 + col % TY;
             if (x < a.g.nx && y < a.g.ny) {
                 const long long row = grid_at(a.g, x, y, z0);
@@ -318,7 +318,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_grid(GridArgs a) {
                     if (z0 + z < a.g.nz) {
 #pragma unroll
                         for (int q = 0; q < CG; ++q)
-                            if (g * CG + q < C) G[row + z * C + g * CG + q] = acc[k][q][z];
+                            if (a.c0 + g * CG + q < C) G[row + z * C + a.c0 + g * CG + q] = acc[k][q][z];
                     }
             }
         }
@@ -460,7 +460,8 @@ void grid_tile_dims(int tcfg, int* tx, int* ty, int* tz) {
 
 template <int KW, int TX, int TY, int TZ, int CG>
 static hipError_t grid_cp(const GridArgs& a, hipStream_t s) {
-    const int cp = pow2_at_least(a.g.C, 64);
+    // more than 32 coils: launches of 32 coils each (launch_grid)
+    const int cp = a.g.C > 32 ? 32 : pow2_at_least(a.g.C, 64);
     const dim3 grid((unsigned)((long long)a.nb * a.t.ntiles));
     switch (cp) {
         case 1: case 2: case 4: case 8:
@@ -468,7 +469,7 @@ static hipError_t grid_cp(const GridArgs& a, hipStream_t s) {
             break;
         case 16: k_grid<KW, TX, TY, TZ, 16, CG><<<grid, CGS_THREADS, 0, s>>>(a); break;
         case 32: k_grid<KW, TX, TY, TZ, 32, CG><<<grid, CGS_THREADS, 0, s>>>(a); break;
-        default: return hipErrorInvalidValue;   // more than 32 coils: not supported by gridding
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -488,13 +489,24 @@ static hipError_t grid_tile(int tcfg, const GridArgs& a, hipStream_t s) {
     }
 }
 
-hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s) {
+static hipError_t grid_coils(int kw, int tcfg, const GridArgs& a, hipStream_t s) {
     switch (kw) {
         case 3: return grid_tile<3>(tcfg, a, s);
         case 5: return grid_tile<5>(tcfg, a, s);
         case 7: return grid_tile<7>(tcfg, a, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+// Coil grids of more than 32 coils are gridded 32 coils per launch (every
+// launch walks the same candidates; a cell's coils are independent sums).
+hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s) {
+    GridArgs b = a;
+    for (b.c0 = 0; b.c0 < a.g.C; b.c0 += 32) {
+        const hipError_t e = grid_coils(kw, tcfg, b, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 template <int KW, int TX, int TY, int TZ>

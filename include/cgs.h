@@ -38,6 +38,8 @@ typedef struct cgs_fft cgs_fft;
 #define CGS_ERR_NONFINITE 4   /* NaN/Inf in q[0] during CG (Reco.cpp:264-268) */
 #define CGS_ERR_MEMORY 5      /* device allocation failed */
 
+#define CGS_MAX_COILS 256     /* coils per context (memory permitting) */
+
 #define CGS_SOLVE_PLAIN 0     /* one bin at a time, host reads the residual every iteration */
 #define CGS_SOLVE_BATCHED 1   /* all bins per launch, device-side loop control, hipGraph replay */
 

@@ -566,12 +566,14 @@ def test_column_pruning_keeps_images(pkg, orc):
     o.exclude_missing_data(P["data"])
 
 
-@pytest.mark.parametrize("ncoil", [1, 5, 12, 20])
+@pytest.mark.parametrize("ncoil", [1, 5, 12, 20, 33, 48, 64])
 def test_coil_counts_off_the_tile_widths(pkg, orc, ncoil):
     """Coil counts that are not multiples of the kernels' coil tiles (one
     coil; 5 and 12: padded to 8 / 16 coils in gridding and a short last
-    8-coil z-pass chunk; 20: padded to 32 with chunks of 8, 8, 4): E^H and E
-    against the FP64 oracle, and the batched CG-SENSE solve."""
+    8-coil z-pass chunk; 20: padded to 32 with chunks of 8, 8, 4; 33, 48, 64:
+    more than 32 coils, gridded 32 coils per launch, the last launch partly
+    padded): E^H and E against the FP64 oracle, and the batched CG-SENSE
+    solve."""
     P = small_problem(orc, ncoil=ncoil, seed=70 + ncoil)
     o = P["oracle"]
     ctx = gpu_context(pkg, P)
