@@ -26,7 +26,7 @@ def test_create_rejects_bad_sizes(pkg):
     assert L.cgs_create(grid, samp, 8, ctypes.byref(h)) == pkg.cgs.CGS_ERR_ARG
     grid = (ctypes.c_int * 3)(128, 128, 96)
     assert L.cgs_create(grid, samp, 0, ctypes.byref(h)) == pkg.cgs.CGS_ERR_ARG
-    assert L.cgs_create(grid, samp, 33, ctypes.byref(h)) == pkg.cgs.CGS_ERR_ARG
+    assert L.cgs_create(grid, samp, 257, ctypes.byref(h)) == pkg.cgs.CGS_ERR_ARG   # above CGS_MAX_COILS
 
 
 def test_fft_rejects_unsupported_length(pkg):
