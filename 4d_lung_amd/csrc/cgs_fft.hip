@@ -500,9 +500,17 @@ __device__ __forceinline__ float2 alt_sign(float2 v, int l) {
 // Strided pass (y or x axis): block = one outer index x CPB consecutive
 // chunks of W columns; lines of length L with stride a.ls.  The first-stage
 // inputs of the next chunk are fetched while the current chunk is transformed.
-// Chunks per block: 4 for passes whose lines stride a megabyte or more (x),
-// 3 otherwise (measured best on the 320x320x224 grid).
-constexpr int plain_cpb(bool far) { return far ? 4 : 3; }
+// Passes whose lines stride a megabyte or more (x) take chunks of 16 columns,
+// 4 per block; the others (y) 32 columns up to 256-point lines (16 beyond),
+// 6 per block (measured best on the 256x256x192 grids of 16 coils; neutral
+// on the 320x320x224 ones).
+#ifndef CGS_PLAIN_CPB_FAR
+#define CGS_PLAIN_CPB_FAR 4
+#endif
+#ifndef CGS_PLAIN_CPB_NEAR
+#define CGS_PLAIN_CPB_NEAR 6
+#endif
+constexpr int plain_cpb(bool far) { return far ? CGS_PLAIN_CPB_FAR : CGS_PLAIN_CPB_NEAR; }
 // Element at a 32-bit byte offset (or a 64-bit element offset) from a
 // block-uniform base: the 32-bit form lets the load use a scalar base address
 // plus one offset register.
@@ -872,7 +880,10 @@ bool fft_len_supported(int L) {
 #ifndef CGS_PLAIN_WB
 #define CGS_PLAIN_WB 16
 #endif
-static constexpr int plain_w(int L) { return L <= 256 ? CGS_PLAIN_WS : CGS_PLAIN_WB; }
+#ifndef CGS_PLAIN_WF
+#define CGS_PLAIN_WF 16
+#endif
+static constexpr int plain_w(int L, bool far) { return far ? CGS_PLAIN_WF : (L <= 256 ? CGS_PLAIN_WS : CGS_PLAIN_WB); }
 
 // Coil chunk of the z pass: 8 coils per LDS tile (small tiles keep four or
 // more blocks resident per CU, which hides the HBM latency better than wider
@@ -885,9 +896,10 @@ int zpass_w(int L, int C) {
 
 static inline bool plain_far(const PlainArgs& a) { return a.ls * (long long)sizeof(float2) >= (1LL << 20); }
 
-template <int L, int DIR, int CPB>
-static void plain_launch_cpb(const PlainArgs& a, dim3 grid, hipStream_t s) {
-    constexpr int W = plain_w(L);
+template <int L, int DIR, bool FAR>
+static void plain_launch_far(const PlainArgs& a, dim3 grid, hipStream_t s) {
+    constexpr int W = plain_w(L, FAR);
+    constexpr int CPB = plain_cpb(FAR);
     constexpr int NS = plan_nstages(L);
     // largest byte offset from a chunk's first column, and from the base of
     // a line group of the first or last stage
@@ -902,17 +914,18 @@ static void plain_launch_cpb(const PlainArgs& a, dim3 grid, hipStream_t s) {
 
 template <int L, int DIR>
 static void plain_launch(const PlainArgs& a, dim3 grid, hipStream_t s) {
-    if (plain_far(a)) plain_launch_cpb<L, DIR, plain_cpb(true)>(a, grid, s);
-    else plain_launch_cpb<L, DIR, plain_cpb(false)>(a, grid, s);
+    if (plain_far(a)) plain_launch_far<L, DIR, true>(a, grid, s);
+    else plain_launch_far<L, DIR, false>(a, grid, s);
 }
 
 void plain_geometry(int L, long long ls, int* W, int* cpb) {
-    *W = plain_w(L);
-    *cpb = plain_cpb(ls * (long long)sizeof(float2) >= (1LL << 20));
+    const bool far = ls * (long long)sizeof(float2) >= (1LL << 20);
+    *W = plain_w(L, far);
+    *cpb = plain_cpb(far);
 }
 
 hipError_t launch_fft_plain(int L, int dir, const PlainArgs& a, long long nouter, hipStream_t s) {
-    const int W = plain_w(L);
+    const int W = plain_w(L, plain_far(a));
     const long long nchunk = (a.ncols + W - 1) / W;
     const int cpb = plain_cpb(plain_far(a));
     const long long ngrp = (nchunk + cpb - 1) / cpb;
