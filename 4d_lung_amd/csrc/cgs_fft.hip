@@ -888,10 +888,16 @@ static constexpr int plain_w(int L, bool far) { return far ? CGS_PLAIN_WF : (L <
 // Coil chunk of the z pass: 8 coils per LDS tile (small tiles keep four or
 // more blocks resident per CU, which hides the HBM latency better than wider
 // tiles with fewer blocks).
+#ifndef CGS_ZPASS_W
+#define CGS_ZPASS_W 8
+#endif
+#ifndef CGS_ZPASS_NT
+#define CGS_ZPASS_NT 256
+#endif
 int zpass_w(int L, int C) {
     (void)L;
     (void)C;
-    return 8;
+    return CGS_ZPASS_W;
 }
 
 static inline bool plain_far(const PlainArgs& a) { return a.ls * (long long)sizeof(float2) >= (1LL << 20); }
@@ -963,20 +969,22 @@ hipError_t launch_fft_z1(int L, int dir, float2* g, long long nlines, const floa
 // z-pass block shape: 8 coils per LDS tile, 256 threads.
 template <int L>
 static hipError_t zpre_w(int W, const ZArgs& a, long long nouter, hipStream_t s) {
+    constexpr int ZW = CGS_ZPASS_W, NT = CGS_ZPASS_NT;
     const dim3 grid((unsigned)nouter);
-    if (W != 8) return hipErrorInvalidValue;
-    if (a.p && a.kft) k_fft_zpre<L, 8, 256, true, true><<<grid, 256, 0, s>>>(a);
-    else if (a.p) k_fft_zpre<L, 8, 256, true, false><<<grid, 256, 0, s>>>(a);
-    else if (a.kft) k_fft_zpre<L, 8, 256, false, true><<<grid, 256, 0, s>>>(a);
-    else k_fft_zpre<L, 8, 256, false, false><<<grid, 256, 0, s>>>(a);
+    if (W != ZW) return hipErrorInvalidValue;
+    if (a.p && a.kft) k_fft_zpre<L, ZW, NT, true, true><<<grid, NT, 0, s>>>(a);
+    else if (a.p) k_fft_zpre<L, ZW, NT, true, false><<<grid, NT, 0, s>>>(a);
+    else if (a.kft) k_fft_zpre<L, ZW, NT, false, true><<<grid, NT, 0, s>>>(a);
+    else k_fft_zpre<L, ZW, NT, false, false><<<grid, NT, 0, s>>>(a);
     return hipGetLastError();
 }
 
 template <int L>
 static hipError_t zpost_w(int W, const ZArgs& a, long long nouter, hipStream_t s) {
+    constexpr int ZW = CGS_ZPASS_W, NT = CGS_ZPASS_NT;
     const dim3 grid((unsigned)nouter);
-    if (W != 8) return hipErrorInvalidValue;
-    k_fft_zpost<L, 8, 256><<<grid, 256, 0, s>>>(a);
+    if (W != ZW) return hipErrorInvalidValue;
+    k_fft_zpost<L, ZW, NT><<<grid, NT, 0, s>>>(a);
     return hipGetLastError();
 }
 
