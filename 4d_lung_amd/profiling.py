@@ -207,7 +207,7 @@ def main(argv):
                 wb = med(write[st]) / nb
                 gbs = (fb + wb) * nb / (med(dur[st]) * 1e-6) / 1e9
                 f.write("%s %.6g %.6g %.0f%s\n" % (st, fb, wb, gbs,
-                                                    "  # above the best copy rate (%.0f GB/s): 64-byte reads" % COPY_GBS
+                                                    "  # above the best copy rate (%.0f GB/s): 64-byte reads or memory-side cache hits" % COPY_GBS
                                                     if gbs > COPY_GBS else ""))
     kpath = os.path.join(out_dir, "kernels_%s.md" % workload)
     head = ""
