@@ -46,6 +46,15 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("libcgs_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    # torch bundles its own HIP runtime (SONAME libamdhip64.so.7, loaded by
+    # the name libamdhip64.so).  Loaded after the library, torch would bring
+    # a second runtime into the process, and the second one to open the GPU
+    # finds none.  With torch loaded first the library binds to torch's copy
+    # by SONAME and the process holds one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     I = ctypes.c_int
