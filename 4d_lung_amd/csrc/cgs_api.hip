@@ -1483,9 +1483,9 @@ long long cgs_device_bytes(const cgs_ctx* h) { return h ? h->dev_bytes : 0; }
 
 // ------------------------------------------------------------------ Voronoi
 // One build of the listed cells (all when list is null; wide: from the wide
-// box of the hull rule): small limits in LDS, short overflow lists again with
-// medium limits in LDS, the rest with large limits in device memory.  Returns
-// the closed cells that outgrew the large limits in *over.
+// box of the hull rule): small limits in LDS, the cells that outgrow them
+// with medium limits in LDS, the rest with large limits in a device
+// workspace.  Returns the closed cells that outgrew the large limits in *over.
 static int voronoi_pass(cgs_ctx* h, const double* dp, long long n, const int* list, bool wide, const VorHash& H,
                         double* dv, int* redo, int* redo2, int* dov, int* over, hipStream_t st) {
     CK(hipMemsetAsync(dov, 0, 3 * sizeof(int), st));
@@ -1493,11 +1493,7 @@ static int voronoi_pass(cgs_ctx* h, const double* dp, long long n, const int* li
     int cnt[3] = {0, 0, 0};
     CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
     CK(hipStreamSynchronize(st));
-    // Middle pass (medium limits in LDS, about five cells resident per CU) for
-    // short lists; its overflows go to the last pass.  Long lists (the long
-    // thin cells of sparse line sets) finish sooner in the last pass, which
-    // builds every listed cell at once from device memory.
-    if (cnt[0] > 0 && cnt[0] <= 6144) {
+    if (cnt[0] > 0) {
         CK(launch_voronoi_mid(dp, wide, H, redo, cnt[0], dv, redo2, dov + 2, st));
         CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
@@ -1505,24 +1501,21 @@ static int voronoi_pass(cgs_ctx* h, const double* dp, long long n, const int* li
         cnt[0] = cnt[2];
     }
     if (cnt[0] > 0) {
-        // last pass: one thread per listed cell, each with a large cell
-        // workspace (about 330 KB) in device memory, as many as a quarter of
-        // the free memory (at most 16 GB) holds
-        size_t cell_b = 0, scr_b = 0;
-        voronoi_large_sizes(&cell_b, &scr_b);
+        // last tier: one wave per cell, each with a large-limit cell (about
+        // 170 KB) in device memory, as many waves as the list has cells, up
+        // to what an eighth of the free memory (at most 4 GB) holds
+        const size_t cell_b = voronoi_large_cell_bytes();
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
-        const long long budget = std::min<long long>((long long)(fr / 4), 16LL << 30);
-        const long long fit = std::max<long long>(2048, budget / (long long)(cell_b + scr_b) / 64 * 64);
-        const int nthreads = (int)std::min<long long>(((long long)cnt[0] + 63) / 64 * 64, fit);
-        char *cells = nullptr, *scr = nullptr;
-        CKS(dalloc(h, &cells, cell_b * nthreads));
-        CKS(dalloc(h, &scr, scr_b * nthreads));
-        CK(launch_voronoi_large(dp, wide, H, redo, cnt[0], nthreads, cells, scr, dv, dov + 1, st));
+        const long long budget = std::min<long long>((long long)(fr / 8), 4LL << 30);
+        const long long fit = std::max<long long>(64, budget / (long long)cell_b);
+        const int nwaves = (int)std::min<long long>((long long)cnt[0], fit);
+        char* cells = nullptr;
+        CKS(dalloc(h, &cells, cell_b * nwaves));
+        CK(launch_voronoi_large(dp, wide, H, redo, cnt[0], nwaves, cells, dv, dov + 1, st));
         CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
-        dfree(h, cells, cell_b * nthreads);
-        dfree(h, scr, scr_b * nthreads);
+        dfree(h, cells, cell_b * nwaves);
     }
     *over += cnt[1];
     return CGS_OK;

@@ -207,20 +207,21 @@ struct VorHash {
 // this factor times their largest extent, visiting every hash cell.
 constexpr double kVorWideBox = 65536.0;
 
-// First pass (small cell limits): volumes of the listed points (list[0..n),
-// or every point when list is null); points whose cell outgrew the limits are
-// appended to redo[*nredo].  wide: build from the wide box (hull rule, second
-// stage).
+// Voronoi cells, one wave per cell (cgs_voronoi_cell.h).  First tier (small
+// limits, LDS): volumes of the listed points (list[0..n), or every point when
+// list is null); points whose cell outgrew the limits are appended to
+// redo[*nredo].  wide: build from the wide box (hull rule, second stage).
 hipError_t launch_voronoi(const double* pts, long long n, const int* list, bool wide, const VorHash& H, double* vol,
                           int* redo, int* nredo, hipStream_t s);
-// Middle pass (medium limits) over redo[0..nredo); overflows go to redo2.
+// Second tier (medium limits, LDS) over redo[0..nredo); overflows go to redo2.
 hipError_t launch_voronoi_mid(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo, double* vol,
                               int* redo2, int* nredo2, hipStream_t s);
-// Last pass (large limits) over redo[0..nredo) with nthreads threads (a
-// multiple of 64), each using one cell and one scratch slot of the
-// workspaces; closed cells that still overflow are counted in *overflow.
-void voronoi_large_sizes(size_t* cell, size_t* scratch);
+// Last tier (large limits) over redo[0..nredo) with nwaves waves, each
+// building its cells in its own slot of the workspace `cells`
+// (voronoi_large_cell_bytes() per wave); closed cells that still overflow
+// are counted in *overflow.
+size_t voronoi_large_cell_bytes();
 hipError_t launch_voronoi_large(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo,
-                                int nthreads, void* cells, void* scratch, double* vol, int* overflow, hipStream_t s);
+                                int nwaves, void* cells, double* vol, int* overflow, hipStream_t s);
 
 }  // namespace cgs
