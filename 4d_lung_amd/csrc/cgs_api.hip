@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -1485,34 +1486,47 @@ long long cgs_device_bytes(const cgs_ctx* h) { return h ? h->dev_bytes : 0; }
 // One build of the listed cells (all when list is null; wide: from the wide
 // box of the hull rule): small limits in LDS, the cells that outgrow them
 // with medium limits in LDS, the rest with large limits in a device
-// workspace.  Returns the closed cells that outgrew the large limits in *over.
+// workspace.  Every tier gives a cell the same bits; the environment variable
+// CGS_VORONOI_FIRST_TIER (1 or 2) starts every cell at a later tier, so that
+// tests can check each tier on its own.  Returns the closed cells that
+// outgrew the large limits in *over.
 static int voronoi_pass(cgs_ctx* h, const double* dp, long long n, const int* list, bool wide, const VorHash& H,
                         double* dv, int* redo, int* redo2, int* dov, int* over, hipStream_t st) {
+    const char* ft = std::getenv("CGS_VORONOI_FIRST_TIER");
+    const int first = ft ? std::atoi(ft) : 0;
     CK(hipMemsetAsync(dov, 0, 3 * sizeof(int), st));
-    CK(launch_voronoi(dp, n, list, wide, H, dv, redo, dov, st));
+    // cells still to build: the list (n points, every point when list is null)
+    long long todo = n;
+    const int* tl = list;
     int cnt[3] = {0, 0, 0};
-    CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
-    CK(hipStreamSynchronize(st));
-    if (cnt[0] > 0) {
-        CK(launch_voronoi_mid(dp, wide, H, redo, cnt[0], dv, redo2, dov + 2, st));
+    if (first <= 0) {
+        CK(launch_voronoi(dp, n, list, wide, H, dv, redo, dov, st));
         CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
+        todo = cnt[0];
+        tl = redo;
         std::swap(redo, redo2);
-        cnt[0] = cnt[2];
     }
-    if (cnt[0] > 0) {
+    if (todo > 0 && first <= 1) {
+        CK(launch_voronoi_mid(dp, todo, tl, wide, H, dv, redo, dov + 2, st));
+        CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        CK(hipStreamSynchronize(st));
+        todo = cnt[2];
+        tl = redo;
+    }
+    if (todo > 0) {
         // last tier: one wave per cell, each with a large-limit cell (about
-        // 170 KB) in device memory, as many waves as the list has cells, up
-        // to what an eighth of the free memory (at most 4 GB) holds
+        // 256 KB) in device memory, as many waves as there are cells, up to
+        // what an eighth of the free memory (at most 4 GB) holds
         const size_t cell_b = voronoi_large_cell_bytes();
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 0;
         const long long budget = std::min<long long>((long long)(fr / 8), 4LL << 30);
         const long long fit = std::max<long long>(64, budget / (long long)cell_b);
-        const int nwaves = (int)std::min<long long>((long long)cnt[0], fit);
+        const int nwaves = (int)std::min<long long>(todo, fit);
         char* cells = nullptr;
         CKS(dalloc(h, &cells, cell_b * nwaves));
-        CK(launch_voronoi_large(dp, wide, H, redo, cnt[0], nwaves, cells, dv, dov + 1, st));
+        CK(launch_voronoi_large(dp, wide, H, tl, todo, nwaves, cells, dv, dov + 1, st));
         CK(hipMemcpyAsync(cnt, dov, sizeof(cnt), hipMemcpyDeviceToHost, st));
         CK(hipStreamSynchronize(st));
         dfree(h, cells, cell_b * nwaves);

@@ -213,15 +213,16 @@ constexpr double kVorWideBox = 65536.0;
 // redo[*nredo].  wide: build from the wide box (hull rule, second stage).
 hipError_t launch_voronoi(const double* pts, long long n, const int* list, bool wide, const VorHash& H, double* vol,
                           int* redo, int* nredo, hipStream_t s);
-// Second tier (medium limits, LDS) over redo[0..nredo); overflows go to redo2.
-hipError_t launch_voronoi_mid(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo, double* vol,
-                              int* redo2, int* nredo2, hipStream_t s);
-// Last tier (large limits) over redo[0..nredo) with nwaves waves, each
+// Second tier (medium limits, LDS), the same over list[0..n) (or every point).
+hipError_t launch_voronoi_mid(const double* pts, long long n, const int* list, bool wide, const VorHash& H,
+                              double* vol, int* redo, int* nredo, hipStream_t s);
+// Last tier (large limits) over redo[0..nredo) (every point when redo is
+// null) with nwaves waves, each
 // building its cells in its own slot of the workspace `cells`
 // (voronoi_large_cell_bytes() per wave); closed cells that still overflow
 // are counted in *overflow.
 size_t voronoi_large_cell_bytes();
-hipError_t launch_voronoi_large(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo,
+hipError_t launch_voronoi_large(const double* pts, bool wide, const VorHash& H, const int* redo, long long nredo,
                                 int nwaves, void* cells, double* vol, int* overflow, hipStream_t s);
 
 }  // namespace cgs

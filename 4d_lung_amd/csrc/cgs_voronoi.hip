@@ -18,12 +18,11 @@
 // unbounded ones) get 0.
 //
 // The 64 lanes of a wave build one cell together: candidate points are
-// tested 64 at a time (one per lane, against every vertex) and each clip
-// runs lane-parallel over the vertex pool and the faces.  Cells live in LDS
-// with small limits (about 14 KB, eleven cells per CU); cells that outgrow
-// them are listed and built again with medium limits (about 28 KB of LDS),
-// and what outgrows those with large limits in a device workspace, one
-// cell per wave.  Built with -ffp-contract=off: every operation is a
+// tested 64 at a time (one per lane) and each clip runs lane-parallel over
+// the vertex pool, the ring entries and the faces.  Cells live in LDS with
+// small limits (16 KB, ten cells per CU); cells that outgrow them are listed
+// and built again with medium limits (31 KB of LDS), and what outgrows those
+// with large limits in a device workspace, one cell per wave.  Built with -ffp-contract=off: every operation is a
 // separately rounded IEEE double operation in a fixed order, so the CPU
 // statement of the same algorithm reproduces the volumes bit for bit, and a
 // cell's bits do not depend on the limits it was built with.
@@ -55,16 +54,17 @@ __global__ __launch_bounds__(64) void k_voronoi_lds(const double* __restrict__ p
 }
 
 // Last tier: large limits, each wave's cell in its own slot of a device
-// workspace.  Closed cells that outgrow these limits too are counted.
+// workspace, over the listed points (every point when redo is null).  Closed
+// cells that outgrow these limits too are counted.
 __global__ __launch_bounds__(64) void k_voronoi_global(const double* __restrict__ p, bool wide, VorHash H,
-                                                      const int* __restrict__ redo, int nredo,
+                                                      const int* __restrict__ redo, long long nredo,
                                                       WCell<LargeLimits>* __restrict__ cells,
                                                       double* __restrict__ vol, int* __restrict__ overflow) {
     WCell<LargeLimits>& C = cells[blockIdx.x];
     const int lane = threadIdx.x;
     WState S;
     for (long long k = blockIdx.x; k < nredo; k += gridDim.x) {
-        const long long i = redo[k];
+        const long long i = redo ? (long long)redo[k] : k;
         const double v = wpoint_volume(p, i, H, C, S, true, wide, lane);
         if (lane == 0) {
             vol[i] = v;
@@ -86,17 +86,16 @@ hipError_t launch_voronoi(const double* pts, long long n, const int* list, bool 
     return hipGetLastError();
 }
 
-hipError_t launch_voronoi_mid(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo, double* vol,
-                              int* redo2, int* nredo2, hipStream_t s) {
-    if (nredo == 0) return hipSuccess;
-    vor::k_voronoi_lds<vor::MediumLimits><<<vor_blocks(nredo), 64, 0, s>>>(pts, nredo, redo, wide, H, vol, redo2,
-                                                                          nredo2);
+hipError_t launch_voronoi_mid(const double* pts, long long n, const int* list, bool wide, const VorHash& H,
+                              double* vol, int* redo, int* nredo, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    vor::k_voronoi_lds<vor::MediumLimits><<<vor_blocks(n), 64, 0, s>>>(pts, n, list, wide, H, vol, redo, nredo);
     return hipGetLastError();
 }
 
 size_t voronoi_large_cell_bytes() { return sizeof(vor::WCell<vor::LargeLimits>); }
 
-hipError_t launch_voronoi_large(const double* pts, bool wide, const VorHash& H, const int* redo, int nredo,
+hipError_t launch_voronoi_large(const double* pts, bool wide, const VorHash& H, const int* redo, long long nredo,
                                 int nwaves, void* cells, double* vol, int* overflow, hipStream_t s) {
     if (nredo == 0 || nwaves <= 0) return hipSuccess;
     vor::k_voronoi_global<<<(unsigned)nwaves, 64, 0, s>>>(pts, wide, H, redo, nredo,

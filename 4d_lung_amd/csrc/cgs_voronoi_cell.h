@@ -6,20 +6,24 @@
 // the bisector half-spaces of the other points, and sum area * |q - p| / 6
 // over the faces.  The 64 lanes of a wave share one cell:
 //
-//  * candidate points come 64 at a time, one per lane, and every lane tests
-//    its point's bisector against all vertices of the cell (broadcast reads);
-//    the points that cut are then clipped one after another, in order;
-//  * a clip runs over the vertex pool (one vertex per lane) and the faces
-//    (one face per lane, walking its ring) with wave prefix sums placing the
-//    results, so the cell stays in a canonical form: kept vertices in pool
-//    order, then one new vertex per distinct cut edge in ascending edge-key
-//    order; faces in order with their rings back to back; the new face's
-//    points sorted by angle through per-lane ranks.
+//  * candidate points come 64 at a time, one per lane; every lane tests its
+//    point's bisector against the cell (a bounding-sphere and bounding-box
+//    bound first, then every vertex by broadcast reads); the points that cut
+//    are then clipped one after another, nearest first;
+//  * a clip runs over the vertex pool (one vertex per lane), over the ring
+//    entries of all faces (one entry per lane) and over the faces, with
+//    wave prefix sums placing the results, so the cell stays in a canonical
+//    form: kept vertices in pool order, then one new vertex per distinct cut
+//    edge in ascending edge-key order; faces in order with their rings back
+//    to back; the new face's points sorted by angle through per-lane ranks.
 //
 // The same statement runs sequentially on the CPU (oracle/voronoi_clip.cpp);
 // every floating-point value is formed by the same separately rounded
 // operations in the same order (the file is compiled with
-// -ffp-contract=off), so the volumes agree bit for bit.
+// -ffp-contract=off), so the volumes agree bit for bit.  The bounding-box
+// test is exact to the vertex test it precedes: with lo <= v <= hi per axis,
+// every rounded product and sum of the bound is >= the same rounded term of
+// n.v, so a bound inside the half-space means every vertex is.
 //
 // Every __syncthreads() and wave vote below sits in control flow that is
 // uniform across the wave: loop bounds and branch conditions are values that
@@ -28,6 +32,7 @@
 #pragma once
 
 #include <cstdint>
+#include <type_traits>
 
 namespace cgs {
 namespace vor {
@@ -35,54 +40,66 @@ namespace vor {
 constexpr int RMAX = 10;    // hash shells visited at most (first box)
 constexpr int WAVE = 64;
 
-// Capacities of a cell: faces, points of the new face, pool vertices and
-// ring entries (all faces' rings back to back), and the pool index type.
-template <int F, int V, int P, int R, class I>
+// Capacities of a cell: faces, points of the new face, pool vertices, ring
+// entries (all faces' rings back to back) and cut-edge occurrences of one
+// clip, and the pool index type.
+template <int F, int V, int P, int R, int O, class I>
 struct Limits {
     static constexpr int FMAX = F;
     static constexpr int VMAX = V;
     static constexpr int PMAX = P;
     static constexpr int NRING = R;
+    static constexpr int OCC = O;
     using Idx = I;
 };
-using SmallLimits = Limits<64, 48, 128, 384, uint8_t>;
-using MediumLimits = Limits<128, 96, 254, 768, uint8_t>;
-using LargeLimits = Limits<384, 256, 2048, 4864, uint16_t>;
+using SmallLimits = Limits<64, 48, 128, 384, 128, uint8_t>;
+using MediumLimits = Limits<128, 96, 254, 768, 256, uint8_t>;
+using LargeLimits = Limits<384, 256, 2048, 4864, 4864, uint16_t>;
 
 struct V3 {
     double x, y, z;
 };
 
 // Cell and clip scratch of one wave.  Face f's ring is
-// ring[cur][fo[cur][f] .. fo[cur][f + 1]); a clip writes the new pool, faces
-// and rings into the other halves.
+// ring[cur][fo[cur][f] .. fo[cur][f + 1]) and efc[cur][e] is the face of ring
+// entry e; a clip writes the new pool, faces and rings into the other halves.
 template <class LM>
 struct WCell {
     using Idx = typename LM::Idx;
-    static constexpr int FMAX = LM::FMAX, VMAX = LM::VMAX, PMAX = LM::PMAX, NRING = LM::NRING;
+    using FIdx = typename std::conditional<(LM::FMAX < 256), uint8_t, uint16_t>::type;
+    static constexpr int FMAX = LM::FMAX, VMAX = LM::VMAX, PMAX = LM::PMAX, NRING = LM::NRING, OCC = LM::OCC;
     V3 v[2][PMAX];
     double fh[2][FMAX];
     int fid[2][FMAX];
     uint16_t fo[2][FMAX + 1];
     Idx ring[2][NRING];
+    FIdx efc[2][NRING];
     double s[PMAX];              // signed distances; face terms of the volume
-    uint32_t key[NRING];         // cut-edge keys in face / ring order
-    uint16_t rank[NRING];        // their new pool indices
-    uint16_t fcnt[FMAX], fkept[FMAX], focc[FMAX];
+    uint16_t eo[NRING + 1];      // outputs of the ring entries before entry e
+    uint16_t eoc[NRING];         // occurrence number of a cut entry
+    uint8_t ef[NRING];           // bit 0: entry's vertex kept, bit 1: entry's edge cut
+    uint32_t key[OCC];           // cut-edge keys in ring order
+    uint16_t rank[OCC];          // their new pool indices
+    uint8_t first[OCC];
+    int16_t nfi[FMAX];           // new index of a surviving face, -1 when dropped
+    int16_t nbase[FMAX];         // its new ring offset less eo at its first entry
     Idx map[PMAX];
     Idx cand[PMAX];
     uint8_t kon[PMAX];
-    uint8_t first[NRING];
     double pa[VMAX], pb[VMAX];
     uint16_t rk[VMAX];
     Idx u[VMAX], sorted[VMAX];
     int hst[WAVE], hoff[WAVE];   // a group of hash cells: first point, offset in the group's list
+    double hd2[WAVE];            // a batch's cutting points: distances in lane order
+    uint8_t hord[WAVE];          // and their lanes nearest first
 };
 
 // The wave's copy of the cell's scalar state (identical in every lane).
+// rmax2 and the bounding box lo/hi are refreshed lazily (dirty after a clip).
 struct WState {
-    int nf, nvert, cur, nbox, overflow;
+    int nf, nvert, nring, cur, nbox, overflow, dirty;
     double rmax2;
+    double lo[3], hi[3];
 };
 
 __device__ __forceinline__ double dot3(const V3& a, const V3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
@@ -91,6 +108,8 @@ __device__ __forceinline__ V3 cross3(const V3& a, const V3& b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
 __device__ __forceinline__ bool same3(const V3& a, const V3& b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 
 __device__ __forceinline__ bool ang_less(double a1, double b1, double a2, double b2) {
     const bool u1 = b1 > 0.0 || (b1 == 0.0 && a1 > 0.0);
@@ -120,14 +139,6 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int* total) {
 // same bits.
 __device__ __forceinline__ double wave_sum(double v) {
     for (int d = WAVE / 2; d >= 1; d >>= 1) v = v + __shfl_xor(v, d);
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v) {
-    for (int d = WAVE / 2; d >= 1; d >>= 1) {
-        const double o = __shfl_xor(v, d);
-        v = o > v ? o : v;
-    }
     return v;
 }
 
@@ -173,7 +184,10 @@ __device__ void winit_box(WCell<LM>& C, WState& S, const double* lo, const doubl
         C.fo[0][lane] = (uint16_t)(4 * lane);
     }
     if (lane == 6) C.fo[0][6] = 24;
-    if (lane < 24) C.ring[0][lane] = q[lane >> 2][lane & 3];
+    if (lane < 24) {
+        C.ring[0][lane] = q[lane >> 2][lane & 3];
+        C.efc[0][lane] = (typename WCell<LM>::FIdx)(lane >> 2);
+    }
     double r2 = 0.0;
     for (int k = 0; k < 8; k++) {
         const V3 c = {(k & 4) ? hi[0] : lo[0], (k & 2) ? hi[1] : lo[1], (k & 1) ? hi[2] : lo[2]};
@@ -182,11 +196,44 @@ __device__ void winit_box(WCell<LM>& C, WState& S, const double* lo, const doubl
     }
     S.nf = 6;
     S.nvert = 8;
+    S.nring = 24;
     S.cur = 0;
     S.nbox = 6;
     S.overflow = 0;
+    S.dirty = 0;
     S.rmax2 = r2;
+    for (int a = 0; a < 3; a++) {
+        S.lo[a] = lo[a];
+        S.hi[a] = hi[a];
+    }
     __syncthreads();
+}
+
+// rmax2 = largest |v|^2 over the pool, lo/hi = its bounding box (exact
+// minima and maxima, so independent of the reduction order).
+template <class LM>
+__device__ void wrefresh(const WCell<LM>& C, WState& S, int lane) {
+    const V3* v = C.v[S.cur];
+    double r2 = 0.0, lx = v[0].x, ly = v[0].y, lz = v[0].z, hx = lx, hy = ly, hz = lz;
+    for (int j = lane; j < S.nvert; j += WAVE) {
+        const V3 x = v[j];
+        r2 = dmax(r2, dot3(x, x));
+        lx = dmin(lx, x.x); ly = dmin(ly, x.y); lz = dmin(lz, x.z);
+        hx = dmax(hx, x.x); hy = dmax(hy, x.y); hz = dmax(hz, x.z);
+    }
+    for (int d = WAVE / 2; d >= 1; d >>= 1) {
+        r2 = dmax(r2, __shfl_xor(r2, d));
+        lx = dmin(lx, __shfl_xor(lx, d));
+        ly = dmin(ly, __shfl_xor(ly, d));
+        lz = dmin(lz, __shfl_xor(lz, d));
+        hx = dmax(hx, __shfl_xor(hx, d));
+        hy = dmax(hy, __shfl_xor(hy, d));
+        hz = dmax(hz, __shfl_xor(hz, d));
+    }
+    S.rmax2 = r2;
+    S.lo[0] = lx; S.lo[1] = ly; S.lo[2] = lz;
+    S.hi[0] = hx; S.hi[1] = hy; S.hi[2] = hz;
+    S.dirty = 0;
 }
 
 // Intersect the cell with the half-space n.x <= |n|^2/2 (nid: the point the
@@ -197,7 +244,8 @@ __device__ void winit_box(WCell<LM>& C, WState& S, const double* lo, const doubl
 template <class LM>
 __device__ void wclip(WCell<LM>& C, WState& S, const V3 n, int nid, int lane) {
     using Idx = typename LM::Idx;
-    constexpr int FMAX = LM::FMAX, VMAX = LM::VMAX, PMAX = LM::PMAX, NRING = LM::NRING;
+    using FIdx = typename WCell<LM>::FIdx;
+    constexpr int FMAX = LM::FMAX, VMAX = LM::VMAX, PMAX = LM::PMAX, NRING = LM::NRING, OCC = LM::OCC;
     constexpr int SH = 8 * (int)sizeof(Idx);
     constexpr uint32_t LOW = (1u << SH) - 1u;
     const double cc = 0.5 * dot3(n, n);
@@ -205,7 +253,7 @@ __device__ void wclip(WCell<LM>& C, WState& S, const V3 n, int nid, int lane) {
     const int cur = S.cur, nxt = cur ^ 1;
     const V3* ov = C.v[cur];
     V3* nvx = C.v[nxt];
-    const int nvert = S.nvert, nf = S.nf;
+    const int nvert = S.nvert, nf = S.nf, nr = S.nring;
 
     // signed distances (lane i % 64 owns vertex i)
     bool out = false;
@@ -230,44 +278,42 @@ __device__ void wclip(WCell<LM>& C, WState& S, const V3 n, int nid, int lane) {
         nk += __popcll(m);
     }
     __syncthreads();
-    // cut-edge occurrences, face by face in ring order (lane f % 64 owns face f)
-    int nocc = 0;
-    for (int f0 = 0; f0 < nf; f0 += WAVE) {
-        const int f = f0 + lane;
-        int cnt = 0, kept = 0, b0 = 0, len = 0;
-        if (f < nf) {
-            b0 = C.fo[cur][f];
-            len = C.fo[cur][f + 1] - b0;
-            for (int k = 0; k < len; k++) {
-                const double sa = C.s[C.ring[cur][b0 + k]];
-                const double sb = C.s[C.ring[cur][b0 + (k + 1 == len ? 0 : k + 1)]];
-                kept += sa <= tol;
-                cnt += is_cut(sa, sb, tol);
-            }
+    // ring entries (lane e % 64 owns entry e): the entry's vertex kept, its
+    // edge to the next entry of the face cut; outputs before each entry; cut
+    // keys in ring order
+    int nocc = 0, nout = 0;
+    for (int e0 = 0; e0 < nr; e0 += WAVE) {
+        const int e = e0 + lane;
+        int kept = 0, cut = 0;
+        uint32_t key = 0;
+        if (e < nr) {
+            const int f = C.efc[cur][e];
+            const int nx = (e + 1 == C.fo[cur][f + 1]) ? C.fo[cur][f] : e + 1;
+            const int a = C.ring[cur][e], b = C.ring[cur][nx];
+            const double sa = C.s[a], sb = C.s[b];
+            kept = sa <= tol;
+            cut = is_cut(sa, sb, tol);
+            const int iu = sa < 0.0 ? a : b;
+            const int iw = sa < 0.0 ? b : a;
+            key = ((uint32_t)iu << SH) | (uint32_t)iw;
         }
+        const uint64_t mc = __ballot(cut);
+        const int oc = nocc + __popcll(mc & lanes_below(lane));
         int tot;
-        const int ex = wave_excl_scan(cnt, lane, &tot);
-        if (f < nf) {
-            C.fcnt[f] = (uint16_t)cnt;
-            C.fkept[f] = (uint16_t)kept;
-            C.focc[f] = (uint16_t)(nocc + ex < NRING ? nocc + ex : NRING);
-            if (nocc + ex + cnt <= NRING) {
-                int j = nocc + ex;
-                for (int k = 0; k < len; k++) {
-                    const int a = C.ring[cur][b0 + k];
-                    const int b = C.ring[cur][b0 + (k + 1 == len ? 0 : k + 1)];
-                    const double sa = C.s[a], sb = C.s[b];
-                    if (is_cut(sa, sb, tol)) {
-                        const int iu = sa < 0.0 ? a : b;
-                        const int iw = sa < 0.0 ? b : a;
-                        C.key[j++] = ((uint32_t)iu << SH) | (uint32_t)iw;
-                    }
-                }
+        const int ex = wave_excl_scan(kept + cut, lane, &tot);
+        if (e < nr) {
+            C.eo[e] = (uint16_t)(nout + ex);
+            C.ef[e] = (uint8_t)(kept | (cut << 1));
+            if (cut) {
+                C.eoc[e] = (uint16_t)oc;
+                if (oc < OCC) C.key[oc] = key;
             }
         }
-        nocc += tot;
+        nocc += __popcll(mc);
+        nout += tot;
     }
-    if (nocc > NRING) { S.overflow = 1; return; }
+    if (lane == 0) C.eo[nr] = (uint16_t)nout;
+    if (nocc > OCC) { S.overflow = 1; return; }
     __syncthreads();
     // one new vertex per distinct key, numbered in ascending key order
     int nnew = 0;
@@ -301,29 +347,29 @@ __device__ void wclip(WCell<LM>& C, WState& S, const V3 n, int nid, int lane) {
         }
     }
     const int nvert2 = nk + nnew;
-    __syncthreads();
-    // faces keeping 3 or more points, in order
+    // faces (lane f % 64 owns face f) keeping 3 or more points, in order
     int nf2 = 0, nr2 = 0, nbox = 0;
     for (int f0 = 0; f0 < nf; f0 += WAVE) {
         const int f = f0 + lane;
-        const int no = f < nf ? C.fkept[f] + C.fcnt[f] : 0;
+        int no = 0, b0 = 0;
+        if (f < nf) {
+            b0 = C.fo[cur][f];
+            no = C.eo[C.fo[cur][f + 1]] - C.eo[b0];
+        }
         const bool keepf = no >= 3;
         const uint64_t m = __ballot(keepf);
         int tot;
         const int ex = wave_excl_scan(keepf ? no : 0, lane, &tot);
-        if (keepf && nr2 + tot <= NRING) {
-            const int nfi = nf2 + __popcll(m & lanes_below(lane));
-            C.fid[nxt][nfi] = C.fid[cur][f];
-            C.fh[nxt][nfi] = C.fh[cur][f];
-            C.fo[nxt][nfi] = (uint16_t)(nr2 + ex);
-            const int b0 = C.fo[cur][f], len = C.fo[cur][f + 1] - b0;
-            int j = nr2 + ex, o = C.focc[f];
-            for (int k = 0; k < len; k++) {
-                const int a = C.ring[cur][b0 + k];
-                const int b = C.ring[cur][b0 + (k + 1 == len ? 0 : k + 1)];
-                const double sa = C.s[a], sb = C.s[b];
-                if (sa <= tol) C.ring[nxt][j++] = C.map[a];
-                if (is_cut(sa, sb, tol)) C.ring[nxt][j++] = (Idx)C.rank[o++];
+        if (f < nf) {
+            if (keepf) {
+                const int nfi = nf2 + __popcll(m & lanes_below(lane));
+                C.nfi[f] = (int16_t)nfi;
+                C.nbase[f] = (int16_t)(nr2 + ex - C.eo[b0]);
+                C.fid[nxt][nfi] = C.fid[cur][f];
+                C.fh[nxt][nfi] = C.fh[cur][f];
+                C.fo[nxt][nfi] = (uint16_t)(nr2 + ex);
+            } else {
+                C.nfi[f] = -1;
             }
         }
         nbox += __popcll(__ballot(keepf && C.fid[cur][f] < 0));
@@ -332,6 +378,24 @@ __device__ void wclip(WCell<LM>& C, WState& S, const V3 n, int nid, int lane) {
     }
     if (nr2 > NRING) { S.overflow = 1; return; }
     __syncthreads();
+    // every entry of a surviving face writes its kept vertex and its cut point
+    for (int e = lane; e < nr; e += WAVE) {
+        const int f = C.efc[cur][e];
+        const int nfi = C.nfi[f];
+        if (nfi >= 0) {
+            const int fl = C.ef[e];
+            int pos = C.nbase[f] + C.eo[e];
+            if (fl & 1) {
+                C.ring[nxt][pos] = C.map[C.ring[cur][e]];
+                C.efc[nxt][pos] = (FIdx)nfi;
+                pos++;
+            }
+            if (fl & 2) {
+                C.ring[nxt][pos] = (Idx)C.rank[C.eoc[e]];
+                C.efc[nxt][pos] = (FIdx)nfi;
+            }
+        }
+    }
     // the new face: on-plane kept and new vertices in pool order, repeated values dropped
     int ncand = 0;
     for (int j0 = 0; j0 < nvert2; j0 += WAVE) {
@@ -420,21 +484,20 @@ __device__ void wclip(WCell<LM>& C, WState& S, const V3 n, int nid, int lane) {
             C.fh[nxt][nf2] = sqrt(dot3(n, n));
             C.fo[nxt][nf2] = (uint16_t)nr2;
         }
-        for (int t = lane; t < nu; t += WAVE) C.ring[nxt][nr2 + t] = C.sorted[t];
+        for (int t = lane; t < nu; t += WAVE) {
+            C.ring[nxt][nr2 + t] = C.sorted[t];
+            C.efc[nxt][nr2 + t] = (FIdx)nf2;
+        }
         nf2++;
         nr2 += nu;
     }
     if (lane == 0) C.fo[nxt][nf2] = (uint16_t)nr2;
-    double r2 = 0.0;
-    for (int j = lane; j < nvert2; j += WAVE) {
-        const double d = dot3(nvx[j], nvx[j]);
-        if (d > r2) r2 = d;
-    }
-    S.rmax2 = wave_max(r2);
     S.cur = nxt;
     S.nvert = nvert2;
     S.nf = nf2;
+    S.nring = nr2;
     S.nbox = nbox;
+    S.dirty = 1;
     __syncthreads();
 }
 
@@ -465,15 +528,14 @@ __device__ double wcell_volume(WCell<LM>& C, const WState& S, int lane) {
     return vol / 6.0;
 }
 
-// Volume of point i's cell (see the header).  wide = false: the points' box
-// widened by one hash cell, at most RMAX shells; wide = true (hull rule,
-// second stage): the box widened by kVorWideBox times the points' extent,
-// every shell.  On return S.overflow tells whether the cell outgrew the
-// limits; with final_pass a cell that still reached the box when it did is
-// taken as unbounded (0, overflow cleared).
-// Hash: the uniform grid over the points (lo, h, n, start, idx; boxes blo/bhi
-// and wlo/whi), hash cell c holding idx[start[c] .. start[c + 1]) in
-// ascending point index.
+// Volume of point i's cell (see the header).  Hash: the uniform grid over the
+// points (lo, h, n, start, idx; boxes blo/bhi and wlo/whi), hash cell c
+// holding idx[start[c] .. start[c + 1]) in ascending point index.  wide =
+// false: the points' box widened by one hash cell, at most RMAX shells; wide
+// = true (hull rule, second stage): the box widened by kVorWideBox times the
+// points' extent, every shell.  On return S.overflow tells whether the cell
+// outgrew the limits; with final_pass a cell that still reached the box when
+// it did is taken as unbounded (0, overflow cleared).
 template <class LM, class Hash>
 __device__ double wpoint_volume(const double* __restrict__ p, long long i, const Hash& H, WCell<LM>& C,
                                 WState& S, bool final_pass, bool wide, int lane) {
@@ -523,9 +585,11 @@ __device__ double wpoint_volume(const double* __restrict__ p, long long i, const
             C.hoff[lane] = ex;
             __syncthreads();
             for (int q0 = 0; q0 < tot; q0 += WAVE) {
+                if (S.dirty) wrefresh(C, S, lane);
                 const int t = q0 + lane;
                 int j = -1;
                 V3 nrm = {0.0, 0.0, 0.0};
+                double d2 = 0.0;
                 bool hit = false;
                 if (t < tot) {
                     // the group's last hash cell starting at or before t
@@ -537,20 +601,41 @@ __device__ double wpoint_volume(const double* __restrict__ p, long long i, const
                     j = H.idx[C.hst[lo] + (t - C.hoff[lo])];
                     const V3 q = {p[3LL * j], p[3LL * j + 1], p[3LL * j + 2]};
                     nrm = sub3(q, pi);
-                    const double d2 = dot3(nrm, nrm);
+                    d2 = dot3(nrm, nrm);
                     // duplicate points (d2 == 0) share their vertex and cut nothing
                     if (j != i && d2 != 0.0 && 0.25 * d2 < S.rmax2) {
                         const double cc = 0.5 * d2;
                         const double tol = 1e-10 * cc;
-                        const V3* vv = C.v[S.cur];
-                        for (int v = 0; v < S.nvert; v++)
-                            if (dot3(nrm, vv[v]) - cc > tol) { hit = true; break; }
+                        const double bound = (dmax(nrm.x * S.lo[0], nrm.x * S.hi[0]) +
+                                              dmax(nrm.y * S.lo[1], nrm.y * S.hi[1])) +
+                                             dmax(nrm.z * S.lo[2], nrm.z * S.hi[2]);
+                        if (bound - cc > tol) {
+                            const V3* vv = C.v[S.cur];
+                            for (int v = 0; v < S.nvert; v++)
+                                if (dot3(nrm, vv[v]) - cc > tol) { hit = true; break; }
+                        }
                     }
                 }
-                uint64_t mask = __ballot(hit);
-                while (mask) {
-                    const int b = __ffsll((long long)mask) - 1;
-                    mask &= mask - 1;
+                const uint64_t mask = __ballot(hit);
+                const int nh = __popcll(mask);
+                if (nh == 0) continue;
+                if (nh > 1) {
+                    // nearest first, ties in batch order
+                    const int hp = __popcll(mask & lanes_below(lane));
+                    if (hit) C.hd2[hp] = d2;
+                    __syncthreads();
+                    if (hit) {
+                        int r = 0;
+                        for (int u = 0; u < nh; u++) {
+                            const double du = C.hd2[u];
+                            r += (du < d2 || (du == d2 && u < hp)) ? 1 : 0;
+                        }
+                        C.hord[r] = (uint8_t)lane;
+                    }
+                    __syncthreads();
+                }
+                for (int o = 0; o < nh; o++) {
+                    const int b = nh == 1 ? __ffsll((long long)mask) - 1 : C.hord[o];
                     const V3 nb = {__shfl(nrm.x, b), __shfl(nrm.y, b), __shfl(nrm.z, b)};
                     const int jb = __shfl(j, b);
                     const bool box = S.nbox > 0;
@@ -566,6 +651,7 @@ __device__ double wpoint_volume(const double* __restrict__ p, long long i, const
             }
             __syncthreads();
         }
+        if (S.dirty) wrefresh(C, S, lane);
         const double reach = (double)r * H.h;
         if (0.25 * reach * reach >= S.rmax2) break;
     }

@@ -19,10 +19,10 @@
 // batch is first tested against the cell as it is at the start of the batch
 // (a point whose bisector leaves every vertex inside cannot cut any later,
 // smaller cell), and the points that cut it are then clipped one after
-// another in batch order (the device kernel takes a group with one hash cell
-// per lane and tests a batch with one point per lane).  A cell that still has a face of the box is taken as
-// unbounded and gets volume 0, as points on the convex hull do in
-// volume_of_finite_dual_cell (functions.h:148-150).  Clipped rule: this also
+// another, nearest first (the device kernel takes a group with one hash cell
+// per lane and tests a batch with one point per lane).  A cell that still
+// has a face of the box is taken as unbounded and gets volume 0, as points
+// on the convex hull do in volume_of_finite_dual_cell (functions.h:148-150).  Clipped rule: this also
 // zeroes the cells of interior points whose Voronoi region reaches more than
 // one hash cell beyond the point cloud (near-degenerate boundary cells whose
 // CGAL volume exceeds the whole k-space).  Hull rule (the reference's): such
@@ -455,24 +455,33 @@ inline void shell_cell(int r, long k, int* dx, int* dy, int* dz) {
 
 // One batch of up to 64 points j[0..nb): every point is tested against the
 // cell as it is at the start of the batch, then the points that cut it are
-// clipped in batch order.  Returns false when the cell outgrew the limits
-// (c.overflow stays set unless a final pass takes a cell that still reaches
-// the box as unbounded, which clears it).
+// clipped nearest first (ascending |q - p|^2, ties in batch order).  Returns
+// false when the cell outgrew the limits (c.overflow stays set unless a
+// final pass takes a cell that still reaches the box as unbounded, which
+// clears it).
 template <class LM>
 bool clip_batch(const double* p, long i, const V3& pi, const int* j, int nb, Cell<LM>& c, ClipScratch<LM>& w,
                 bool final_pass) {
-    bool hit[BATCH];
     V3 nn[BATCH];
+    double d2[BATCH];
+    int hit[BATCH];
+    int nh = 0;
     const double rm = c.rmax2;
     for (int t = 0; t < nb; t++) {
         const V3 q = {p[3L * j[t]], p[3L * j[t] + 1], p[3L * j[t] + 2]};
         nn[t] = sub3(q, pi);
-        const double d2 = dot3(nn[t], nn[t]);
+        d2[t] = dot3(nn[t], nn[t]);
         // duplicate points (d2 == 0) share their vertex and cut nothing
-        hit[t] = j[t] != i && d2 != 0.0 && 0.25 * d2 < rm && cuts(c, nn[t]);
+        if (j[t] != i && d2[t] != 0.0 && 0.25 * d2[t] < rm && cuts(c, nn[t])) hit[nh++] = t;
     }
-    for (int t = 0; t < nb; t++) {
-        if (!hit[t]) continue;
+    for (int a = 1; a < nh; a++) {   // stable insertion sort by distance
+        const int x = hit[a];
+        int b = a - 1;
+        while (b >= 0 && d2[hit[b]] > d2[x]) { hit[b + 1] = hit[b]; b--; }
+        hit[b + 1] = x;
+    }
+    for (int h = 0; h < nh; h++) {
+        const int t = hit[h];
         const bool box = has_box_face(c);
         clip(c, w, nn[t], j[t]);
         if (c.overflow) {

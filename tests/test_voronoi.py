@@ -169,6 +169,36 @@ def test_device_voronoi_bit_exact(pkg, orc, case, rule):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tier", [1, 2])
+def test_device_voronoi_every_tier_bit_exact(pkg, orc, tier, monkeypatch):
+    """Cells built from the start with the medium (LDS) or the large (device
+    memory) limits give the same bits as the oracle: a cell's bits do not
+    depend on the limits it is built with."""
+    monkeypatch.setenv("CGS_VORONOI_FIRST_TIER", str(tier))
+    for case, p, rule in (("sparse_bin", sparse_bin_points()[:6000], "clipped"), ("lattice", lattice(8), "hull")):
+        print("device Voronoi %s, %s rule, %d points, first tier %d" % (case, rule, len(p), tier), flush=True)
+        np.testing.assert_array_equal(pkg.cgs.voronoi_volumes(p, rule=rule), orc.voronoi_volumes(p, rule=rule))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wl,b", [("4d_128x128x96", 7), ("4d_256x256x160", 0)])
+def test_device_voronoi_workload_bin_bit_exact(pkg, orc, wl, b):
+    """Per-bin weights of the respiratory-resolved workloads (the clipped rule
+    on one bin's acquired samples): long thin cells, a tenth of them built
+    again with the medium limits on the 256x256x160 workload."""
+    cfg = synth.CONFIGS[wl]
+    grid, samp = cfg["grid"], cfg["samp"]
+    traj = synth.wave_caipi_trajectory(grid, samp, cfg["cycles"], cfg["amp"])
+    lines = synth.caipi_mask(samp[1], samp[2], cfg["R"], cfg["shift"])
+    labels = synth.respiratory_bins(lines, cfg["bins"], cfg["bin_frac"], cfg["seed"] + 5)
+    p = np.ascontiguousarray(traj[synth.sample_mask_from_lines(labels == b, samp[0])])
+    print("device Voronoi %s bin %d, %d points" % (wl, b, len(p)), flush=True)
+    got = pkg.cgs.voronoi_volumes(p, rule="clipped")
+    np.testing.assert_array_equal(got, orc.voronoi_volumes(p, rule="clipped"))
+    assert 0.7 < (got > 0).mean() < 0.95
+
+
+@pytest.mark.gpu
 def test_bin_voronoi_dcf_in_cg(pkg, orc):
     from problems import gpu_context, small_problem
 
