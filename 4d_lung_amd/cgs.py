@@ -85,6 +85,7 @@ def lib():
         "cgs_set_bin_group": (I, [P, I]),
         "cgs_set_pruning": (I, [P, I]),
         "cgs_set_grid_lists": (I, [P, I]),
+        "cgs_set_zpass_bins": (I, [P, I]),
         "cgs_get_pruning": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_get_prune_fractions": (I, [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "cgs_get_bin_group": (I, [P, ctypes.POINTER(ctypes.c_int)]),
@@ -295,6 +296,11 @@ class Context:
         """Per-tile gridding candidate lists built when the bins are packed
         (True, the default) or found by a scan in every gridding launch."""
         self._ck(self._L.cgs_set_grid_lists(self.h, int(bool(on))))
+
+    def set_zpass_bins(self, bins):
+        """Bins per workgroup of the batched inverse z pass (1: one
+        workgroup per bin and cell line; default 64)."""
+        self._ck(self._L.cgs_set_zpass_bins(self.h, int(bins)))
 
     def pruning(self):
         """Whether the packed bins use column pruning."""

@@ -94,6 +94,7 @@ struct cgs_ctx {
     uint8_t* d_tneed = nullptr;    // [nbins][nty*ntz]
     // per-tile gridding candidate lists of the packed bins (built with them)
     int grid_lists = 1;            // 1: build and use them; 0: gridding scans every launch
+    int zbpw = 64;                 // batched inverse z pass: bins per workgroup and cell line
     int* d_tlist = nullptr;
     long long tlist_n = 0;
     long long* d_tlist_off = nullptr;   // [nbins * ntiles + 1]
@@ -682,7 +683,7 @@ int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const 
     ZArgs z{};
     z.g = h->d_grid; z.tw = h->d_tw[2]; z.p = p; z.maps = p ? h->d_maps : nullptr;
     z.kft = deapod ? h->d_kft : nullptr; z.C = h->C; z.ng = h->ng; z.nxy = (long long)h->N[0] * h->N[1];
-    z.xp = h->xp; z.bp = h->bp; z.ny = h->N[1]; z.done = done; z.nb = nb;
+    z.xp = h->xp; z.bp = h->bp; z.ny = h->N[1]; z.done = done; z.nb = nb; z.bpw = h->zbpw;
     {
         Prof t(h, "fft_z_fwd");
         CK(launch_fft_zpre(h->N[2], W, z, (long long)nb * z.nxy, h->st));
@@ -735,7 +736,7 @@ int run_inv_fft(cgs_ctx* h, int b0, int nb, float2* q, const float2* p, double* 
     z.q = q; z.part = part;
     z.scale = (float)(1e-3 / std::sqrt((double)h->ng));
     z.C = h->C; z.ng = h->ng; z.nxy = (long long)h->N[0] * h->N[1]; z.ny = h->N[1]; z.done = done;
-    z.xp = h->xp; z.bp = h->bp; z.nb = nb;
+    z.xp = h->xp; z.bp = h->bp; z.nb = nb; z.bpw = h->zbpw;
     bool crop = false;
     for (int a = 0; a < 3; ++a) {
         if (h->imgRange[a] != h->N[a]) crop = true;
@@ -1155,6 +1156,13 @@ int cgs_set_grid_lists(cgs_ctx* h, int on) {
     if (on == h->grid_lists) return CGS_OK;
     h->grid_lists = on;
     free_packed(h);
+    return CGS_OK;
+}
+
+int cgs_set_zpass_bins(cgs_ctx* h, int bins) {
+    if (!h || bins < 1 || bins > 64) return CGS_ERR_ARG;
+    h->zbpw = bins;
+    drop_graph(h);   // the captured iteration holds the z-pass launches
     return CGS_OK;
 }
 

@@ -46,6 +46,7 @@ struct ZArgs {
     int lo[3], hi[3];      // kept box, inclusive on both ends
     const int* done;       // per-bin flags of the launch's bins, or null
     int nb;                // bins of the launch (workgroup order, zpass_line)
+    int bpw;               // bins per workgroup of the inverse z pass (k_fft_zpost_mb; < 2: k_fft_zpost)
 };
 
 bool fft_len_supported(int L);

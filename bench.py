@@ -148,6 +148,8 @@ def bench_binned(args, cfg, pkg, torch):
     voronoi = voronoi_s is not None
     counts = [int(bb[0].size) for bb in bins]
     ctx.set_grid_lists(args.grid_lists)
+    if args.zpass_bins > 0:
+        ctx.set_zpass_bins(args.zpass_bins)
     ctx.set_num_bins(B)
 
     def load_bins():
@@ -352,6 +354,8 @@ def main():
     ap.add_argument("--grid-lists", type=int, default=1, choices=(0, 1),
                     help="per-tile gridding candidate lists built at packing (1, the library's default) or a scan "
                          "in every gridding launch (0)")
+    ap.add_argument("--zpass-bins", type=int, default=0,
+                    help="bins per workgroup of the batched inverse z pass (0: the library's default, 64)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (the reference kernel that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run)")

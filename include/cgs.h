@@ -129,6 +129,13 @@ int cgs_get_prune_fractions(const cgs_ctx* h, int bin, double* groups, double* t
  * candidates in the same order, so the same images bit for bit.  Takes effect
  * when the bins are next packed.  No reference counterpart. */
 int cgs_set_grid_lists(cgs_ctx* h, int on);
+/* Inverse z pass of batched launches (the fused IFFT along z, deapodization
+ * and coil combination of E^H): bins per workgroup, 1..64 (default 64).
+ * With 2 or more, one workgroup takes one cell line for that many bins of
+ * the launch, the line's coil maps held in registers (up to 16 coils) and
+ * read once for them all; with 1, one workgroup per bin and cell line.  The
+ * same images bit for bit.  No reference counterpart. */
+int cgs_set_zpass_bins(cgs_ctx* h, int bins);
 /* Compact form: m acquired samples with original indices idx[m] (any order),
  * values y[c][m] complex and optional weights dcf[m]. */
 int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, const float* y, const float* dcf);

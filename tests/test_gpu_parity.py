@@ -359,6 +359,47 @@ def test_batched_bins_match_plain_and_oracle(pkg, orc):
     o.exclude_missing_data(P["data"])
 
 
+@pytest.mark.parametrize("coils", [5, 12, 16])
+def test_inverse_zpass_bin_groups_bit_identical(pkg, orc, coils):
+    """The inverse z pass taking one cell line for several bins of a launch
+    (the coil maps held once for all of them; 2, 3 and 64 bins per workgroup
+    over five bins, one of them empty, so finished after its first pass and
+    skipped from then on) gives the images, residuals and pass counts of the
+    per-bin workgroups bit for bit, and the oracle's."""
+    P = small_problem(orc, seed=61, ncoil=coils)
+    o = P["oracle"]
+    nb = 5
+    lines = P["lines"]
+    acq = np.flatnonzero(lines.ravel())
+    labels = np.full(lines.size, -1)
+    labels[acq] = np.array([0, 1, 3, 4])[np.arange(acq.size) % 4]   # bin 2 has no lines
+    labels = labels.reshape(lines.shape)
+    ctx = gpu_context(pkg, P)
+    ctx.set_num_bins(nb)
+    ctx.set_bin_group(nb)
+    bins = []
+    for b in range(nb):
+        sm = np.broadcast_to((labels == b)[None], (P["samp"][0],) + lines.shape).reshape(-1)
+        d = np.where(sm[None, :], P["data"], 0).astype(np.complex64)
+        ctx.set_bin_data(b, d)
+        bins.append(d)
+    runs = {}
+    for bpw in (1, 2, 3, 64):
+        ctx.set_zpass_bins(bpw)
+        runs[bpw] = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=6)
+    for bpw in (2, 3, 64):
+        for k in range(3):
+            np.testing.assert_array_equal(runs[bpw][k], runs[1][k])
+    x, _, its = runs[64]
+    assert its[2] == 1 and all(its[b] == 7 for b in (0, 1, 3, 4))
+    for b in (0, nb - 1):
+        o.calculate_weights(bins[b])
+        xr, _, _ = o.cg_sense(bins[b], thresh=0.0, max_iter=6, prec=64)
+        assert rel_l2(x[b], xr) < CG_TOL
+    o.calculate_weights()
+    o.exclude_missing_data(P["data"])
+
+
 def test_empty_bin_follows_the_reference(pkg, orc):
     """A bin without acquired samples between two ordinary bins.  In
     Reco::cgSenseDynamic its right-hand side is 0, so rr = pq = 0 and
