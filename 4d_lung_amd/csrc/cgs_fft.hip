@@ -647,10 +647,12 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_z1(float2* g, long long nli
 // bin s % nb, s = k / 8: the nb workgroups of one cell line have ids 8 apart,
 // run on one XCD close together in time, and all but the first find that
 // line's maps in L2.  A bijection onto bin * nxy + xy whenever nxy is a
-// multiple of 8; otherwise (and for one bin) the bin-major order.
-__device__ __forceinline__ long long zpass_line(long long k, long long nxy, int nb) {
+// multiple of 8; otherwise (and for one bin) the bin-major order.  32-bit
+// arithmetic (nb * nxy < 2^32): a workgroup computes its indices before its
+// first load, and 64-bit divisions there cost a long scalar sequence.
+__device__ __forceinline__ unsigned zpass_line(unsigned k, unsigned nxy, unsigned nb) {
     if (nb <= 1 || (nxy & 7)) return k;
-    const long long s = k >> 3;
+    const unsigned s = k >> 3;
     return (s % nb) * nxy + (s / nb) * 8 + (k & 7);
 }
 
@@ -666,10 +668,11 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     __shared__ float2 twl[L];
     __shared__ float2 pline[HAS_P ? L : 1];   // p along the line (shared by all coils)
     __shared__ float kline[HAS_K ? L : 1];    // kernelFT along the line
-    const long long outer = zpass_line(blockIdx.x, a.nxy, a.nb);   // bin * nxy + xy
-    const long long bin = outer / a.nxy, xy = outer % a.nxy;
-    if (a.done && a.done[bin]) return;
-    const long long cell0 = xy * L;                  // cell index of (x, y, z=0)
+    const unsigned nxy = (unsigned)a.nxy, ny = (unsigned)a.ny;
+    const unsigned outer = zpass_line(blockIdx.x, nxy, (unsigned)a.nb);   // bin * nxy + xy
+    const unsigned bin = outer / nxy, xy = outer - bin * nxy;
+    const unsigned gx = xy / ny, gy = xy - gx * ny;
+    const long long cell0 = (long long)xy * L;       // cell index of (x, y, z=0)
     for (int e = threadIdx.x; e < L; e += NT) {
         twl[e] = a.tw[e];
         if constexpr (HAS_P) pline[e] = a.p[bin * a.ng + cell0 + e];
@@ -678,7 +681,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     // block-uniform bases and 32-bit offsets within the line; coil indices
     // are clamped so that every load is unconditional (columns past the last
     // coil are zeroed when the tile is written to LDS)
-    float2* g = a.g + bin * a.bp + (xy / a.ny) * a.xp + (xy % a.ny) * (long long)(L * a.C);
+    float2* g = a.g + bin * a.bp + gx * a.xp + gy * (long long)(L * a.C);
     const int C = a.C;
     const float2* ml = HAS_P ? a.maps + cell0 * C : nullptr;
     // raw element: S (with p) or the coil image value (without)
@@ -699,6 +702,10 @@ __global__ __launch_bounds__(NT) void k_fft_zpre(ZArgs a) {
     };
     TileRegs<L, W, NT, float2> next;
     next.load(ld(0));
+    // a finished bin's workgroup leaves after issuing its loads (reading a
+    // finished bin is harmless; checking first would put the flag's latency
+    // in front of every load)
+    if (a.done && a.done[bin]) return;
     __syncthreads();   // pline / kline / twl visible
     // The outputs of an even chunk are held in registers and stored together
     // with the next chunk's, so that both halves of a cell's 16-coil line are
@@ -755,15 +762,15 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
     __shared__ float2 twl[L];
     __shared__ float kline[L];      // kernelFT * scale along the line
     __shared__ double red[2 * (NT / 64)];
-    const long long outer = zpass_line(blockIdx.x, a.nxy, a.nb);   // bin * nxy + xy
-    const long long bin = outer / a.nxy, xy = outer % a.nxy;
-    if (a.done && a.done[bin]) return;
-    const long long cell0 = xy * L;
+    const unsigned nxy = (unsigned)a.nxy, ny = (unsigned)a.ny;
+    const unsigned outer = zpass_line(blockIdx.x, nxy, (unsigned)a.nb);   // bin * nxy + xy
+    const unsigned bin = outer / nxy, xy = outer - bin * nxy;
+    const long long cell0 = (long long)xy * L;
     for (int e = threadIdx.x; e < L; e += NT) {
         twl[e] = a.tw[e];
         kline[e] = a.kft ? a.kft[cell0 + e] * a.scale : a.scale;
     }
-    const int gx = (int)(xy / a.ny), gy = (int)(xy % a.ny);
+    const int gx = (int)(xy / ny), gy = (int)(xy - (unsigned)gx * ny);
     const bool xy_in = !a.crop || (gx >= a.lo[0] && gx <= a.hi[0] && gy >= a.lo[1] && gy <= a.hi[1]);
     float2* g = a.g + bin * a.bp + gx * a.xp + gy * (long long)(L * a.C);
     const bool combine = a.maps != nullptr;
@@ -782,6 +789,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
     };
     TileRegs<L, W, NT, float2> next;
     next.load(ld(0));
+    if (a.done && a.done[bin]) return;   // after the loads are issued (see k_fft_zpre)
     for (int c0 = 0; c0 < C; c0 += W) {
         cur = c0;
         next.template to_lds<LDW>(lds, cv);
@@ -855,8 +863,8 @@ __global__ __launch_bounds__(NT) void k_fft_zpost(ZArgs a) {
         if (threadIdx.x == 0) {
             double sr = 0.0, si = 0.0;
             for (int w = 0; w < NT / 64; ++w) { sr += red[2 * w]; si += red[2 * w + 1]; }
-            a.part[2 * outer] = sr;
-            a.part[2 * outer + 1] = si;
+            a.part[2 * (long long)outer] = sr;
+            a.part[2 * (long long)outer + 1] = si;
         }
     }
 }
@@ -885,15 +893,15 @@ __device__ __forceinline__ uint64_t active_bins(const int* done, int nb, int g0,
 // workgroups of one line get ids 8 apart (one XCD, close in time, so the
 // line's maps are read from L2 after the first) whenever nxy is a multiple
 // of 8, as zpass_line does for single bins.
-__device__ __forceinline__ long long zpass_group_line(long long k, long long nxy, int nb, int bpw, int* g0,
-                                                      int* g1) {
-    const int ng = (nb + bpw - 1) / bpw;
-    long long xy, g;
+__device__ __forceinline__ unsigned zpass_group_line(unsigned k, unsigned nxy, int nb, int bpw, int* g0,
+                                                     int* g1) {
+    const unsigned ng = (unsigned)((nb + bpw - 1) / bpw);
+    unsigned xy, g;
     if (ng <= 1 || (nxy & 7)) {
-        xy = k % nxy;
         g = k / nxy;
+        xy = k - g * nxy;
     } else {
-        const long long s = k >> 3;
+        const unsigned s = k >> 3;
         g = s % ng;
         xy = (s / ng) * 8 + (k & 7);
     }
@@ -922,13 +930,13 @@ __global__ __launch_bounds__(NT) void k_fft_zpost_mb(ZArgs a) {
     __shared__ float kline[L];      // kernelFT * scale along the line
     __shared__ double red[2 * (NT / 64)];
     int g0, g1;
-    const long long xy = zpass_group_line(blockIdx.x, a.nxy, a.nb, a.bpw, &g0, &g1);
-    const long long cell0 = xy * L;
+    const unsigned xy = zpass_group_line(blockIdx.x, (unsigned)a.nxy, a.nb, a.bpw, &g0, &g1);
+    const long long cell0 = (long long)xy * L;
     for (int e = threadIdx.x; e < L; e += NT) {
         twl[e] = a.tw[e];
         kline[e] = a.kft ? a.kft[cell0 + e] * a.scale : a.scale;
     }
-    const int gx = (int)(xy / a.ny), gy = (int)(xy % a.ny);
+    const int gx = (int)(xy / (unsigned)a.ny), gy = (int)(xy - (unsigned)gx * (unsigned)a.ny);
     const bool xy_in = !a.crop || (gx >= a.lo[0] && gx <= a.hi[0] && gy >= a.lo[1] && gy <= a.hi[1]);
     const long long gline = gx * a.xp + gy * (long long)(L * a.C);
     const int C = a.C;
@@ -1022,7 +1030,7 @@ __global__ __launch_bounds__(NT) void k_fft_zpost_mb(ZArgs a) {
                 for (int w = 0; w < NT / 64; ++w) { sr += red[2 * w]; si += red[2 * w + 1]; }
                 const long long outer = (long long)b * a.nxy + xy;
                 a.part[2 * outer] = sr;
-                a.part[2 * outer + 1] = si;
+                a.part[2 * (long long)outer + 1] = si;
             }
         }
         b = bn;

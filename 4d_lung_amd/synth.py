@@ -45,13 +45,13 @@ CONFIGS = {
     "unrolled_96x96x64": dict(image=(96, 96, 64), grid=(192, 192, 128), samp=(192, 96, 64), coils=8,
                               R=(2, 2), shift=1, cycles=3, amp=4.0, iters=5, bins=1, seed=1004),
     # Large 4D precision stress: 256x256x160 image, 32 coils, 20 bins of 4-9 % of
-    # lines each, R=3x3, 512 samples/line, 30 CG iterations.  check_iters: the
-    # FP64 oracle's solve of a bin takes about half a minute per iteration on
-    # 16 host cores, so its accuracy check runs 4 iterations on both sides.
-    # No density weights (the reference's default, Density::NONE).
+    # lines each, R=3x3, 512 samples/line, per-bin Voronoi DCF, 30 CG
+    # iterations.  check_iters: the FP64 oracle's solve of a bin takes about
+    # half a minute per iteration on 16 host cores, so its accuracy check runs
+    # 4 iterations on both sides.
     "4d_256x256x160": dict(image=(256, 256, 160), grid=(512, 512, 320), samp=(512, 256, 160), coils=32,
                            R=(3, 3), shift=1, cycles=6, amp=4.0, iters=30, bins=20, bin_frac=(0.04, 0.09),
-                           seed=1005, check_iters=4, check_bins="0", dcf="none"),
+                           seed=1005, check_iters=4, check_bins="0", dcf="voronoi"),
 }
 
 

@@ -144,9 +144,11 @@ def test_4d_256x256x160_adjoint_identity(pkg, c5):
 
 def test_4d_256x256x160_bin_solve_matches_oracle(pkg, orc, c5):
     """One bin of the 32-coil workload: k-space of the phantom from the
-    library's E on the bin's samples plus noise, then two CG-SENSE passes on
-    the GPU (batched mode, column pruning as the library chooses) against
-    the FP64 oracle's two passes on the same float32 data."""
+    library's E on the bin's samples plus noise and the bin's Voronoi
+    density weights (clipped rule, computed on the device), then two
+    CG-SENSE passes on the GPU (batched mode, column pruning as the library
+    chooses) against the FP64 oracle's two passes on the same float32 data
+    and weights."""
     cfg = c5["cfg"]
     grid, samp, C = cfg["grid"], cfg["samp"], cfg["coils"]
     ctx = _c5_context(pkg, c5, with_maps=True)
@@ -155,7 +157,10 @@ def test_4d_256x256x160_bin_solve_matches_oracle(pkg, orc, c5):
     data = synth.add_noise(y, c5["bmask"], 1e-3, cfg["seed"] + 3)
     del y
     idx = np.flatnonzero(c5["bmask"]).astype(np.int32)
-    ctx.set_bin_samples(0, idx, np.ascontiguousarray(data[:, idx]))
+    t0 = time.time()
+    w = pkg.cgs.voronoi_density(c5["traj"][idx], rule="clipped")
+    print("4d_256x256x160: Voronoi weights of %d samples %.2fs" % (idx.size, time.time() - t0), flush=True)
+    ctx.set_bin_samples(0, idx, np.ascontiguousarray(data[:, idx]), w)
     print("4d_256x256x160: GPU solve (%d samples per coil)" % idx.size, flush=True)
     x, res, its = ctx.cg_sense_bins(mode=pkg.SOLVE_BATCHED, thresh=0.0, max_iter=1)
     ctx.close()
@@ -165,6 +170,9 @@ def test_4d_256x256x160_bin_solve_matches_oracle(pkg, orc, c5):
     o.calculate_weights(data)
     o.calculate_kernel_ft()
     o.set_maps(c5["maps"])
+    wf = np.zeros(int(np.prod(samp)), np.float32)
+    wf[idx] = w
+    o.set_density(wf)
     print("4d_256x256x160: oracle set up %.0fs" % (time.time() - t0), flush=True)
     xr, resr, itr = o.cg_sense(data, 0.0, 1, prec=64)
     del o
