@@ -3,8 +3,9 @@
 Both choices are measured on the caller's own data, not modelled:
 
 * tile shape -- every gridding tile configuration runs one batched CG
-  iteration over all bins; the sum of the gridding and degridding kernel times
-  (HIP events on the library's stream) decides;
+  iteration over all bins; the sum of the kernel times that depend on it
+  (gridding or the inverse x pass with gridding fused in, and degridding;
+  HIP events on the library's stream) decides;
 * bins per launch -- the batched launches cover groups of bins (as many as
   device memory holds coil grids for, or fewer); each candidate group size
   runs a few captured iterations, timed end to end.
@@ -16,6 +17,17 @@ import math
 import time
 
 from . import cgs
+
+
+def tile_stages_ms(rep):
+    """Time of the stages the gridding tile shape affects in a profile
+    report: degridding (samples in tile order) and gridding, or the inverse
+    x pass that has it fused in."""
+    ms = rep["degrid"][1]
+    for name in ("grid", "fft_x_inv_grid"):
+        if name in rep:
+            ms += rep[name][1]
+    return ms
 
 
 def tune_tile(ctx, reload, log=None):
@@ -36,9 +48,9 @@ def tune_tile(ctx, reload, log=None):
         ctx.cg_iterate(cgs.SOLVE_BATCHED, 1)
         rep = ctx.profile_report()
         ctx.profile(False)
-        timings[t] = rep["grid"][1] + rep["degrid"][1]
+        timings[t] = tile_stages_ms(rep)
         if log:
-            log("tile config %d %s: grid+degrid %.3f ms per iteration" % (t, ctx.tile_config()[1], timings[t]))
+            log("tile config %d %s: gridding+degridding %.3f ms per iteration" % (t, ctx.tile_config()[1], timings[t]))
     best = min(timings, key=timings.get)
     ctx.set_tile_config(best)
     reload()

@@ -37,6 +37,7 @@ SOURCES = [
     ("cgs_voronoi.hip", ["-ffp-contract=off"]),
     ("cgs_api.hip", ["-ffp-contract=off"]),
     ("cgs_noalloc.hip", ["-ffp-contract=off"]),
+    ("cgs_xlists.hip", []),
 ]
 
 

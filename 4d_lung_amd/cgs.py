@@ -86,6 +86,8 @@ def lib():
         "cgs_set_pruning": (I, [P, I]),
         "cgs_set_grid_lists": (I, [P, I]),
         "cgs_set_zpass_bins": (I, [P, I]),
+        "cgs_set_fused_gridding": (I, [P, I]),
+        "cgs_fused_gridding_active": (I, [P, P]),
         "cgs_get_pruning": (I, [P, ctypes.POINTER(ctypes.c_int)]),
         "cgs_get_prune_fractions": (I, [P, I, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
         "cgs_get_bin_group": (I, [P, ctypes.POINTER(ctypes.c_int)]),
@@ -296,6 +298,17 @@ class Context:
         """Per-tile gridding candidate lists built when the bins are packed
         (True, the default) or found by a scan in every gridding launch."""
         self._ck(self._L.cgs_set_grid_lists(self.h, int(bool(on))))
+
+    def set_fused_gridding(self, on):
+        """Gridding fused into the inverse x pass for the packed bins (True,
+        the default, where supported) or a separate gridding pass."""
+        self._ck(self._L.cgs_set_fused_gridding(self.h, int(bool(on))))
+
+    def fused_gridding(self):
+        """Whether the packed bins use the fused gridding."""
+        v = ctypes.c_int(0)
+        self._ck(self._L.cgs_fused_gridding_active(self.h, ctypes.byref(v)))
+        return bool(v.value)
 
     def set_zpass_bins(self, bins):
         """Bins per workgroup of the batched inverse z pass (1: one

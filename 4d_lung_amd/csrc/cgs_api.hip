@@ -98,6 +98,12 @@ struct cgs_ctx {
     int* d_tlist = nullptr;
     long long tlist_n = 0;
     long long* d_tlist_off = nullptr;   // [nbins * ntiles + 1]
+    // per-(y, z)-cell sample lists of the fused gridding (k_fft_xgrid)
+    int xgrid = 1;                 // 1: gridding fused into the inverse x pass where supported
+    bool xl_on = false;            // lists built for the packed bins
+    int* d_xidx = nullptr;         // sorted global sample indices
+    long long xl_n = 0;
+    long long* d_xoff = nullptr;   // [nbins][ny * nz + 1]
     std::vector<int> h_gcnt;
     std::vector<double> pr_gfrac, pr_tfrac;   // per bin: groups transformed, tile columns written
 
@@ -389,6 +395,10 @@ void free_packed(cgs_ctx* h) {
     dfree(h, h->d_tlist, std::max(1LL, h->tlist_n));
     dfree(h, h->d_tlist_off, (long long)h->nbins * h->tiles.ntiles + 1);
     h->tlist_n = 0;
+    dfree(h, h->d_xidx, std::max(1LL, h->xl_n));
+    dfree(h, h->d_xoff, (long long)h->nbins * ((long long)h->N[1] * h->N[2] + 1));
+    h->xl_n = 0;
+    h->xl_on = false;
     h->prune_on = false;
     h->pr_ngrp = h->pr_gstride = 0;
     h->h_gcnt.clear();
@@ -501,6 +511,57 @@ int build_grid_lists(cgs_ctx* h) {
     g.tlist_off = h->d_tlist_off;
     CK(launch_grid_lists(h->kb.kw, h->tcfg, g, nullptr, h->st));
     CK(hipStreamSynchronize(h->st));
+    return CGS_OK;
+}
+
+// Per-(y, z)-cell sample lists of the fused gridding (cgs_xlists.hip), for
+// every packed bin; only where the inverse x pass takes 16-coil chunks of a
+// cell (xgrid_supported) and the bins are sparse enough to gain: the fused
+// pass stages each sample once per (y, z) of its window and 16-coil group
+// (16 values and a few factors: about (8 + 3) * C bytes per (y, z)), where
+// the separate pass writes and reads the coil grids (16 * C * Ng bytes);
+// the fused pass is used when the densest bin stages at most a quarter of
+// that (respiratory bins of a few percent of the lines stage a twentieth to
+// a fifth; a single frame with all lines about three quarters, which
+// measured slower fused).
+int build_xlists(cgs_ctx* h) {
+    h->xl_on = false;
+    if (!h->xgrid || !xgrid_supported(h->N[0], h->C, h->xp) || !xlist_supported(h->N[0], h->N[1], h->N[2]))
+        return CGS_OK;
+    const int kw = h->kb.kw;
+    const long long ncell = (long long)h->N[1] * h->N[2];
+    long long total = 0, mmax = 0;
+    for (int b = 0; b < h->nbins; ++b) {
+        const long long n = (h->bin_off[b + 1] - h->bin_off[b]) * kw * kw;
+        total += n;
+        mmax = std::max(mmax, n);
+    }
+    const double staged = (double)mmax * 11.0 * h->C, round_trip = 16.0 * h->C * (double)h->ng;
+    if (staged > 0.25 * round_trip) return CGS_OK;
+    h->xl_n = total;
+    CKS(dalloc(h, &h->d_xidx, std::max(1LL, total)));
+    CKS(dalloc(h, &h->d_xoff, h->nbins * (ncell + 1)));
+    unsigned *ktmp = nullptr, *kout = nullptr;
+    int* itmp = nullptr;
+    char* stmp = nullptr;
+    const size_t sb = std::max<size_t>(1, xlist_sort_bytes(mmax));
+    CKS(dalloc(h, &ktmp, std::max(1LL, mmax)));
+    CKS(dalloc(h, &kout, std::max(1LL, mmax)));
+    CKS(dalloc(h, &itmp, std::max(1LL, mmax)));
+    CKS(dalloc(h, &stmp, sb));
+    long long g0 = 0;
+    for (int b = 0; b < h->nbins; ++b) {
+        const long long m = h->bin_off[b + 1] - h->bin_off[b];
+        CK(build_xlists_bin(kw, h->p_base, h->p_vals, h->bin_off[b], m, h->N[1], h->N[2], ktmp, itmp, stmp, sb, kout,
+                            h->d_xidx + g0, h->d_xoff + b * (ncell + 1), g0, h->st));
+        g0 += m * kw * kw;
+    }
+    CK(hipStreamSynchronize(h->st));
+    dfree(h, ktmp, std::max(1LL, mmax));
+    dfree(h, kout, std::max(1LL, mmax));
+    dfree(h, itmp, std::max(1LL, mmax));
+    dfree(h, stmp, sb);
+    h->xl_on = true;
     return CGS_OK;
 }
 
@@ -635,6 +696,7 @@ int pack_bins(cgs_ctx* h) {
     CK(hipStreamSynchronize(h->st));
     CKS(build_prune(h));
     if (h->grid_lists) CKS(build_grid_lists(h));
+    CKS(build_xlists(h));
     h->packed = true;
     h->cg_ready = false;
     return CGS_OK;
@@ -713,13 +775,30 @@ int run_fwd_fft(cgs_ctx* h, int b0, int nb, const float2* p, bool deapod, const 
 // Inverse x, y passes and the fused inverse z pass.  With q != null the
 // coils are combined with conj(S) into q (and <p, q> partials when p and part
 // are given); otherwise the grids receive the coil images.
+// With xg_y != null the coil grids are not read: the inverse x pass forms
+// them from the samples (gridding fused in, k_fft_xgrid, xg_y the sample
+// values, xg_dcf optional weights; the packed bins' cell lists must exist).
 int run_inv_fft(cgs_ctx* h, int b0, int nb, float2* q, const float2* p, double* part, const int* done,
-                bool bins = true) {
+                bool bins = true, const float2* xg_y = nullptr, const float* xg_dcf = nullptr) {
     PlainArgs x{};
     x.g = h->d_grid; x.os = h->bp; x.ls = h->xp; x.ncols = (long long)h->N[1] * h->N[2] * h->C;
     x.tw = h->d_tw[0]; x.outer_per_bin = 1; x.done = done;
     if (bins && h->prune_on) x.gact = h->d_gact + (size_t)b0 * h->pr_ngrp;
-    {
+    if (xg_y) {
+        XGridArgs xa{};
+        xa.x = x;
+        xa.Y = xg_y;
+        xa.gdcf = xg_dcf;
+        xa.vals = h->p_vals;
+        xa.base = h->p_base;
+        xa.idx = h->d_xidx;
+        xa.off = h->d_xoff;
+        xa.C = h->C; xa.ny = h->N[1]; xa.nz = h->N[2];
+        xa.thrf = h->kb.thr_up;
+        xa.b0 = b0;
+        Prof t(h, "fft_x_inv_grid");
+        CK(launch_fft_xgrid(h->N[0], h->kb.kw, xa, nb, h->st));
+    } else {
         Prof t(h, "fft_x_inv");
         CK(launch_fft_plain(h->N[0], +1, x, nb, h->st));
     }
@@ -787,6 +866,10 @@ int apply_normal_bins(cgs_ctx* h, int b0, int nb, bool use_done) {
     {
         Prof t(h, "degrid");
         CK(launch_degrid(h->kb.kw, g, h->st));
+    }
+    if (h->xl_on) {   // gridding fused into the inverse x pass
+        CKS(run_inv_fft(h, b0, nb, h->d_q + (size_t)b0 * h->ng, p, h->d_part, done, true, h->p_yw));
+        return CGS_OK;
     }
     {
         Prof t(h, "grid");
@@ -1159,6 +1242,20 @@ int cgs_set_grid_lists(cgs_ctx* h, int on) {
     return CGS_OK;
 }
 
+int cgs_set_fused_gridding(cgs_ctx* h, int on) {
+    if (!h || on < 0 || on > 1) return CGS_ERR_ARG;
+    if (on == h->xgrid) return CGS_OK;
+    h->xgrid = on;
+    free_packed(h);
+    return CGS_OK;
+}
+
+int cgs_fused_gridding_active(cgs_ctx* h, int* on) {
+    if (!h || !on) return CGS_ERR_ARG;
+    *on = h->xl_on ? 1 : 0;
+    return CGS_OK;
+}
+
 int cgs_set_zpass_bins(cgs_ctx* h, int bins) {
     if (!h || bins < 1 || bins > 64) return CGS_ERR_ARG;
     h->zbpw = bins;
@@ -1264,11 +1361,15 @@ int cgs_cg_prepare(cgs_ctx* h, double thresh, int max_iter) {
         GridArgs g = grid_args(h, b0, nb);
         g.Y = h->p_y;
         g.gdcf = h->p_dcf;
-        {
-            Prof t(h, "grid");
-            CK(launch_grid(h->kb.kw, h->tcfg, g, h->st));
+        if (h->xl_on) {   // gridding fused into the inverse x pass, as in the iterations
+            CKS(run_inv_fft(h, b0, nb, h->d_r + o, nullptr, nullptr, nullptr, true, h->p_y, h->p_dcf));
+        } else {
+            {
+                Prof t(h, "grid");
+                CK(launch_grid(h->kb.kw, h->tcfg, g, h->st));
+            }
+            CKS(run_inv_fft(h, b0, nb, h->d_r + o, nullptr, nullptr, nullptr));
         }
-        CKS(run_inv_fft(h, b0, nb, h->d_r + o, nullptr, nullptr, nullptr));
         CK(launch_copy_rhs(h->d_x + o, h->d_r + o, h->d_p + o, h->d_r + o, h->ng, nb, h->st));
         CK(launch_dot_partials(h->d_r + o, h->d_r + o, h->ng, nb, h->nblk_upd, h->d_part2, h->st));
         CK(launch_cg_init(h->d_sc + b0, h->d_part2, h->nblk_upd, nb, thresh, h->d_done + b0, h->st));

@@ -621,6 +621,137 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
     }
 }
 
+// Inverse x pass with the gridding of E^H fused in (columns = coil groups of
+// W = 16 coils of one (y, z) cell, C a multiple of 16).  Instead of reading a
+// gridded chunk, the block forms it: the value of line x, coil c of cell
+// (y, z) is the sum over the cell's listed samples with bx <= x < bx + KW
+// (cgs_xlists.hip, sorted by (bx, sample)) of w * Y[s][c] (times gdcf[s]
+// when given), w = float((vx * vy) * vz), added when w >= thrf -- the
+// weights and the tap set of gridding (Reco.cpp:492-504; Reco.h:218-242).
+// The cell's listed samples are staged in LDS, in the memory of the FFT tile
+// before the tile is filled: per sample its bx, its KW weights along x for
+// this cell and its 16 coil values; each first-stage input of a thread then
+// finds its samples by a binary search on bx and sums them from LDS.  The
+// chunk runs through the same stages and stores as the plain inverse x pass.
+// Column groups that no sample reaches (a.gact) are written as zeros.
+template <int L, int W, int CPB, int AM, int KW>
+__global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
+    const PlainArgs& a = xa.x;
+    constexpr int VS = val_stride(KW);
+    constexpr int NS = plan_nstages(L);
+    static_assert(NS > 1, "fused gridding needs a multi-stage line transform");
+    static_assert(W == 16, "fused gridding stages 16 coil values per sample");
+    using FR = FirstRegs<L, W, CGS_THREADS>;
+    constexpr int R0 = FR::R;
+    constexpr int SL = plan_ls(L, NS - 1);
+    // staged samples per batch: bx (int), KW weights (float), W values (float2)
+    constexpr int NEB = (L * W * 8) / (4 + 4 * KW + 8 * W);
+    constexpr int NE = (NEB / 16) * 16 < CGS_THREADS ? (NEB / 16) * 16 : CGS_THREADS;
+    static_assert(NE >= 16, "tile too small to stage samples");
+    __shared__ float2 lds[L * W];
+    __shared__ float2 twl[L];
+    float2* const ey = lds;                                        // [NE][W]
+    float* const ew = reinterpret_cast<float*>(lds + NE * W);      // [NE][KW]
+    int* const ebx = reinterpret_cast<int*>(ew + NE * KW);         // [NE]
+    const long long nchunk = (a.ncols + W - 1) / W;
+    const long long ngrp = (nchunk + CPB - 1) / CPB;
+    const long long grp = blockIdx.x % ngrp, outer = blockIdx.x / ngrp;   // outer: bin of the launch
+    if (a.done && a.done[outer]) return;
+    float2* gout = a.g + outer * a.os;
+    const long long ls = a.ls;
+    const long long c_first = grp * CPB;
+    const long long c_end = min(c_first + CPB, nchunk);
+    if (a.gact && !a.gact[outer * ngrp + grp]) {   // no sample reaches these columns: zeros
+        for (long long chunk = c_first; chunk < c_end; ++chunk) {
+            float2* g = gout + chunk * W;
+            const int cm = (int)min((long long)W, a.ncols - chunk * W) - 1;
+            for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
+                const int l = e / W, c = e % W;
+                if (c <= cm) g[(long long)l * ls + c] = make_float2(0.0f, 0.0f);
+            }
+        }
+        return;
+    }
+    for (int e = threadIdx.x; e < L; e += CGS_THREADS) twl[e] = a.tw[e];
+    const int ncell = xa.ny * xa.nz;
+    const long long* __restrict__ offb = xa.off + (long long)(xa.b0 + outer) * (ncell + 1);
+    const int tid = threadIdx.x;
+    for (long long chunk = c_first; chunk < c_end; ++chunk) {
+        const long long col0 = chunk * W;
+        const int cell = (int)(col0 / xa.C), c0 = (int)(col0 % xa.C);
+        const int y = cell / xa.nz, z = cell - (cell / xa.nz) * xa.nz;
+        const long long e0 = offb[cell], e1 = offb[cell + 1];
+        FR next;
+#pragma unroll
+        for (int p = 0; p < FR::PER; ++p)
+#pragma unroll
+            for (int i = 0; i < R0; ++i) next.v[p][i] = make_float2(0.0f, 0.0f);
+        for (long long eb = e0; eb < e1; eb += NE) {
+            const int ne = (int)min((long long)NE, e1 - eb);
+            // stage: weights along x of this cell (one thread per sample), then
+            // the coil values (W threads per sample, coalesced)
+            if (tid < ne) {
+                const int sidx = xa.idx[eb + tid];
+                const int4 b = xa.base[sidx];
+                const double* v = xa.vals + (long long)sidx * VS;
+                const double vy = v[KW + (y - b.y)], vz = v[2 * KW + (z - b.z)];
+                ebx[tid] = b.x;
+#pragma unroll
+                for (int ox = 0; ox < KW; ++ox) ew[tid * KW + ox] = (float)((v[ox] * vy) * vz);
+            }
+            for (int t = tid; t < ne * W; t += CGS_THREADS) {
+                const int e = t / W, c = t % W;
+                const int sidx = xa.idx[eb + e];
+                float2 yv = xa.Y[(long long)sidx * xa.C + c0 + c];
+                if (xa.gdcf) yv = cscale(yv, xa.gdcf[sidx]);
+                ey[t] = yv;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int p = 0; p < FR::PER; ++p) {
+                const int b = tid + p * CGS_THREADS;
+                if ((FR::NB % CGS_THREADS) == 0 || b < FR::NB) {
+                    const int col = b % W, j = b / W;
+                    int lo = 0;
+#pragma unroll
+                    for (int i = 0; i < R0; ++i) {
+                        const int x = j + i * (L / R0);
+                        // first staged sample with bx >= x - KW + 1 (lines ascend in i)
+                        int hi = ne;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (ebx[mid] < x - KW + 1) lo = mid + 1; else hi = mid;
+                        }
+                        float2 acc = next.v[p][i];
+                        for (int e = lo; e < ne && ebx[e] <= x; ++e) {
+                            const float w = ew[e * KW + (x - ebx[e])];
+                            if (w >= xa.thrf) {
+                                const float2 yv = ey[e * W + col];
+                                acc.x = fmaf(w, yv.x, acc.x);
+                                acc.y = fmaf(w, yv.y, acc.y);
+                            }
+                        }
+                        next.v[p][i] = acc;
+                    }
+                }
+            }
+            __syncthreads();   // staged samples read (the memory is the tile's)
+        }
+        const int cmax = (int)min((long long)W, a.ncols - col0) - 1;
+        next.template run<+1, W>(lds, [&](int l, int c, float2 v) {
+            if (c > cmax) v = make_float2(0.0f, 0.0f);
+            return alt_sign(v, l);
+        });
+        __syncthreads();
+        fft_mid_stages<L, 1, NS - 1, R0, +1, W, W, CGS_THREADS>(lds, twl);
+        float2* g = gout + chunk * W;
+        fft_last_stage<L, +1, W, W, false, CGS_THREADS>(lds, twl, [&](int l, int c, float2 v, int, int i) {
+            if (c <= cmax) *line_at<AM>(g, ls, i, SL, l - i * SL, c) = alt_sign(v, l);
+        });
+        __syncthreads();   // the tile read before the next chunk's samples are staged in it
+    }
+}
+
 // z pass with a single column (one complex grid, C = 1): block = OB lines.
 template <int L, int OB, int DIR>
 __global__ __launch_bounds__(CGS_THREADS) void k_fft_z1(float2* g, long long nlines, const float2* __restrict__ tw) {
@@ -1124,6 +1255,49 @@ hipError_t launch_fft_plain(int L, int dir, const PlainArgs& a, long long nouter
 #undef CGS_CASE
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+template <int L, int KW>
+static void xgrid_launch(const XGridArgs& xa, dim3 grid, hipStream_t s) {
+    constexpr int W = 16, CPB = plain_cpb(true);
+    constexpr int NS = plan_nstages(L);
+    const PlainArgs& a = xa.x;
+    const long long span = ((long long)(L - 1) * a.ls + W) * (long long)sizeof(float2);
+    constexpr int SG = L / plan_radix(L, 0) > plan_ls(L, NS - 1) ? L / plan_radix(L, 0) : plan_ls(L, NS - 1);
+    const long long seg = ((long long)(SG - 1) * a.ls + W) * (long long)sizeof(float2);
+    (void)span;
+    if (seg < (1LL << 32)) k_fft_xgrid<L, W, CPB, kOffSeg, KW><<<grid, CGS_THREADS, 0, s>>>(xa);
+    else k_fft_xgrid<L, W, CPB, kOff64, KW><<<grid, CGS_THREADS, 0, s>>>(xa);
+}
+
+bool xgrid_supported(int L, int C, long long ls) {
+    return (L == 64 || L == 96 || L == 128 || L == 160 || L == 192 || L == 224 || L == 256 || L == 320 ||
+            L == 384 || L == 448 || L == 512) && C % 16 == 0 && plain_far(PlainArgs{nullptr, 0, ls}) &&
+           plain_w(L, true) == 16;
+}
+
+hipError_t launch_fft_xgrid(int L, int kw, const XGridArgs& xa, long long nouter, hipStream_t s) {
+    if (!xgrid_supported(L, xa.C, xa.x.ls)) return hipErrorInvalidValue;
+    const long long nchunk = (xa.x.ncols + 15) / 16;
+    const long long ngrp = (nchunk + plain_cpb(true) - 1) / plain_cpb(true);
+    const dim3 grid((unsigned)(nouter * ngrp));
+#define CGS_XG(n, k) \
+    case n: xgrid_launch<n, k>(xa, grid, s); break;
+#define CGS_XG_KW(k)                                                                            \
+    switch (L) {                                                                                \
+        CGS_XG(64, k) CGS_XG(96, k) CGS_XG(128, k) CGS_XG(160, k) CGS_XG(192, k) CGS_XG(224, k)  \
+        CGS_XG(256, k) CGS_XG(320, k) CGS_XG(384, k) CGS_XG(448, k) CGS_XG(512, k)              \
+        default: return hipErrorInvalidValue;                                                   \
+    }
+    switch (kw) {
+        case 3: CGS_XG_KW(3); break;
+        case 5: CGS_XG_KW(5); break;
+        case 7: CGS_XG_KW(7); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef CGS_XG_KW
+#undef CGS_XG
     return hipGetLastError();
 }
 

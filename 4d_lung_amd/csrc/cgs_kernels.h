@@ -49,6 +49,24 @@ struct ZArgs {
     int bpw;               // bins per workgroup of the inverse z pass (k_fft_zpost_mb; < 2: k_fft_zpost)
 };
 
+// Inverse x pass with gridding fused in (k_fft_xgrid): the x pass of the
+// launch's bins (x.outer = bin), each chunk formed from its cell's listed
+// samples instead of read.
+struct XGridArgs {
+    PlainArgs x;
+    const float2* Y;        // [Mtot][C] sample values
+    const float* gdcf;      // multiply Y by these weights first (or null), as gridding does
+    const double* vals;     // [Mtot][VS] per-axis KB factors
+    const int4* base;       // [Mtot] window origins
+    const int* idx;         // sorted per-cell lists: global sample indices
+    const long long* off;   // [nbins_total][ny * nz + 1] list positions
+    int C, ny, nz;
+    float thrf;             // KbParams::thr_up
+    int b0;                 // first bin of the launch
+};
+bool xgrid_supported(int L, int C, long long ls);
+hipError_t launch_fft_xgrid(int L, int kw, const XGridArgs& xa, long long nouter, hipStream_t s);
+
 bool fft_len_supported(int L);
 int zpass_w(int L, int C);
 hipError_t launch_fft_plain(int L, int dir, const PlainArgs& a, long long nouter, hipStream_t s);
@@ -136,6 +154,21 @@ hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s);
 // counts != null the number per tile goes to counts[bin * ntiles + tile];
 // otherwise the lists are written to a.tlist at a.tlist_off.
 hipError_t launch_grid_lists(int kw, int tcfg, const GridArgs& a, int* counts, hipStream_t s);
+
+// Per-cell sample lists of the fused gridding (cgs_xlists.hip): keys
+// (y * nz + z) << kXListBits | (bx + kXListOff), sorted, with the samples'
+// global indices; off[bin][cell] (absolute positions) of ny * nz + 1 cells.
+constexpr int kXListBits = 12;
+constexpr int kXListOff = 16;
+bool xlist_supported(int nx, int ny, int nz);
+// hipCUB radix-sort workspace for n keys
+size_t xlist_sort_bytes(long long n);
+// The lists of one bin: its m samples from global index first; keys_tmp /
+// idx_tmp of m * kw * kw entries; the sorted keys and indices go to
+// keys_out / idx_out (same size) and the cell offsets, plus off0, to off.
+hipError_t build_xlists_bin(int kw, const int4* base, const double* vals, long long first, long long m, int ny, int nz,
+                            unsigned* keys_tmp, int* idx_tmp, void* sort_tmp, size_t sort_bytes, unsigned* keys_out,
+                            int* idx_out, long long* off, long long off0, hipStream_t s);
 
 // Coordinate gradient of the sample values: for every sample i (sorted
 // position s, original index base[s].w)

@@ -56,6 +56,8 @@ def algorithmic_bytes(name, ng, C, m, vs, touched=None, groups=1.0, tiles=1.0):
         "fft_z_fwd": (16 * C + 12) * ng,             # p, S_c, kft in; coil grids out
         "fft_y_fwd": 16 * C * ng, "fft_x_fwd": 16 * C * ng * groups,
         "fft_x_inv": 8 * C * ng * (1.0 + groups), "fft_y_inv": 16 * C * ng,
+        # inverse x pass with gridding fused in: the samples in, every column out
+        "fft_x_inv_grid": 8 * C * ng + m * (16 + 8 * vs + 8 * C),
         "fft_z_inv": (16 * C + 20) * ng,             # grids, S_c, kft, p in; q out
         "degrid": 8 * C * nt + m * (16 + 8 * vs + 4 + 8 * C),
         "grid": 8 * C * ng * tiles + m * (16 + 8 * vs + 8 * C),
@@ -150,6 +152,7 @@ def bench_binned(args, cfg, pkg, torch):
     ctx.set_grid_lists(args.grid_lists)
     if args.zpass_bins > 0:
         ctx.set_zpass_bins(args.zpass_bins)
+    ctx.set_fused_gridding(args.fused_gridding)
     ctx.set_num_bins(B)
 
     def load_bins():
@@ -354,6 +357,9 @@ def main():
     ap.add_argument("--grid-lists", type=int, default=1, choices=(0, 1),
                     help="per-tile gridding candidate lists built at packing (1, the library's default) or a scan "
                          "in every gridding launch (0)")
+    ap.add_argument("--fused-gridding", type=int, default=1, choices=(0, 1),
+                    help="gridding fused into the inverse x pass where supported (1, the library's default) or a "
+                         "separate gridding pass (0)")
     ap.add_argument("--zpass-bins", type=int, default=0,
                     help="bins per workgroup of the batched inverse z pass (0: the library's default, 64)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -391,6 +397,7 @@ def main():
     grid, samp, image, C = cfg["grid"], cfg["samp"], cfg["image"], cfg["coils"]
     ng, ns = int(np.prod(grid)), int(np.prod(samp))
     K, W = args.steps, args.warmup
+    tune = importlib.import_module("4d_lung_amd.autotune")
 
     t0 = time.time()
     traj = synth.wave_caipi_trajectory(grid, samp, cfg["cycles"], cfg["amp"])
@@ -404,6 +411,7 @@ def main():
     if args.pruning >= 0:
         ctx.set_pruning(args.pruning)
     ctx.set_grid_lists(args.grid_lists)
+    ctx.set_fused_gridding(args.fused_gridding)
     ctx.set_trajectory(traj)
     ctx.calculate_kernel_ft()
     ctx.set_maps(maps)
@@ -436,8 +444,8 @@ def main():
             ctx.apply_normal(p)
             rep = ctx.profile_report()
             ctx.profile(False)
-            ms = rep["grid"][1] + rep["degrid"][1]
-            log("tile config %d %s: grid+degrid %.3f ms" % (t, ctx.tile_config()[1], ms))
+            ms = tune.tile_stages_ms(rep)
+            log("tile config %d %s: gridding+degridding %.3f ms" % (t, ctx.tile_config()[1], ms))
             if best is None or ms < best[1]:
                 best = (t, ms)
         tcfg = best[0]

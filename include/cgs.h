@@ -136,6 +136,20 @@ int cgs_set_grid_lists(cgs_ctx* h, int on);
  * read once for them all; with 1, one workgroup per bin and cell line.  The
  * same images bit for bit.  No reference counterpart. */
 int cgs_set_zpass_bins(cgs_ctx* h, int bins);
+/* Gridding fused into the inverse x pass (on = 1, the default): for the CG
+ * iterations and right-hand sides of the packed bins, the inverse x pass
+ * forms each 16-coil chunk of a (y, z) cell column from the samples whose KB
+ * window covers the cell (lists sorted by window origin, built when the bins
+ * are packed) instead of reading a gridded grid -- the same taps and weights
+ * as gridding, summed in a different order.  Used where the x pass takes
+ * 16-coil chunks (coils a multiple of 16, x planes of 1 MB or more) and the
+ * densest bin stages at most a quarter of the bytes of the coil grids' write
+ * and read (each sample once per (y, z) of its window: sparse respiratory
+ * bins, not a fully sampled frame); takes effect when the bins are next
+ * packed.  No reference counterpart. */
+int cgs_set_fused_gridding(cgs_ctx* h, int on);
+/* *on = 1 when the packed bins use the fused gridding. */
+int cgs_fused_gridding_active(cgs_ctx* h, int* on);
 /* Compact form: m acquired samples with original indices idx[m] (any order),
  * values y[c][m] complex and optional weights dcf[m]. */
 int cgs_set_bin_samples(cgs_ctx* h, int bin, long long m, const int32_t* idx, const float* y, const float* dcf);

@@ -359,6 +359,71 @@ def test_batched_bins_match_plain_and_oracle(pkg, orc):
     o.exclude_missing_data(P["data"])
 
 
+@pytest.fixture(scope="module")
+def wide16(orc):
+    """16 coils on a 64x128x96 grid: x planes of 1.5 MB, so the inverse x
+    pass takes 16-coil chunks of one (y, z) cell and gridding is fused into
+    it for packed bins."""
+    return small_problem(orc, grid=(64, 128, 96), image=(32, 64, 48), samp=(64, 64, 48), ncoil=16, R=(2, 2),
+                         cycles=3, amp=3.0, seed=71, dcf=True)
+
+
+def test_fused_gridding_matches_separate_pass_and_oracle(pkg, orc, wide16):
+    """Six bins (line subsets, own density weights; sparse enough for the
+    fused pass) solved with the gridding fused into the inverse x pass (cell
+    lists sorted by window origin, a gather instead of the tile scatter) and
+    with the separate gridding pass: the same taps and weights summed in
+    another order, so the images agree to FP32 rounding; batched and plain
+    give the same bits with the fused pass; the first and last bins match
+    the FP64 oracle."""
+    P = wide16
+    o = P["oracle"]
+    nb = 6
+    lines = P["lines"]
+    acq = np.flatnonzero(lines.ravel())
+    labels = np.full(lines.size, -1)
+    labels[acq] = np.arange(acq.size) % nb
+    labels = labels.reshape(lines.shape)
+    rng = np.random.default_rng(72)
+    bins = []
+    for b in range(nb):
+        sm = np.broadcast_to((labels == b)[None], (P["samp"][0],) + lines.shape).reshape(-1)
+        d = np.where(sm[None, :], P["data"], 0).astype(np.complex64)
+        w = rng.uniform(0.5, 1.5, sm.size).astype(np.float32)
+        bins.append((d, w))
+    res = {}
+    for fused in (1, 0):
+        ctx = gpu_context(pkg, P)
+        ctx.set_fused_gridding(fused)
+        ctx.set_num_bins(nb)
+        for b, (d, w) in enumerate(bins):
+            ctx.set_bin_data(b, d, w)
+        xb, rb, ib = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=7)
+        assert ctx.fused_gridding() == bool(fused)
+        xp, rp, ip = ctx.cg_sense_bins(mode=0, thresh=0.0, max_iter=7)
+        print("fused gridding %d: batched/plain max rel diff %.2e" % (fused, max(rel_l2(xb[b], xp[b]) for b in range(nb))),
+              flush=True)
+        if fused:
+            np.testing.assert_array_equal(xb, xp)
+        res[fused] = xb
+        ctx.close()
+    for b in range(nb):
+        assert rel_l2(res[1][b], res[0][b]) < 1e-5
+    for b in (0, nb - 1):
+        d, w = bins[b]
+        e_sep = rel_l2(res[1][b], res[0][b])
+        o.calculate_weights(d)
+        o.set_density(w)
+        xr, _, _ = o.cg_sense(d, thresh=0.0, max_iter=7, prec=64)
+        e_or = rel_l2(res[1][b], xr)
+        print("bin %d: fused vs separate %.2e, fused vs FP64 oracle %.2e" % (b, e_sep, e_or), flush=True)
+        assert e_sep < 1e-5
+        assert e_or < CG_TOL
+    o.set_density(P["dcf"])
+    o.calculate_weights()
+    o.exclude_missing_data(P["data"])
+
+
 @pytest.mark.parametrize("coils", [5, 12, 16])
 def test_inverse_zpass_bin_groups_bit_identical(pkg, orc, coils):
     """The inverse z pass taking one cell line for several bins of a launch
