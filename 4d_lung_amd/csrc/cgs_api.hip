@@ -101,7 +101,7 @@ struct cgs_ctx {
     // per-(y, z)-cell sample lists of the fused gridding (k_fft_xgrid)
     int xgrid = 1;                 // 1: gridding fused into the inverse x pass where supported
     bool xl_on = false;            // lists built for the packed bins
-    int* d_xidx = nullptr;         // sorted global sample indices
+    int* d_xrec = nullptr;         // records of the sorted entries (xrec_stride(kw) ints each)
     long long xl_n = 0;
     long long* d_xoff = nullptr;   // [nbins][ny * nz + 1]
     std::vector<int> h_gcnt;
@@ -395,7 +395,7 @@ void free_packed(cgs_ctx* h) {
     dfree(h, h->d_tlist, std::max(1LL, h->tlist_n));
     dfree(h, h->d_tlist_off, (long long)h->nbins * h->tiles.ntiles + 1);
     h->tlist_n = 0;
-    dfree(h, h->d_xidx, std::max(1LL, h->xl_n));
+    dfree(h, h->d_xrec, std::max(1LL, h->xl_n * xrec_stride(h->kb.kw)));
     dfree(h, h->d_xoff, (long long)h->nbins * ((long long)h->N[1] * h->N[2] + 1));
     h->xl_n = 0;
     h->xl_on = false;
@@ -518,8 +518,9 @@ int build_grid_lists(cgs_ctx* h) {
 // every packed bin; only where the inverse x pass takes 16-coil chunks of a
 // cell (xgrid_supported) and the bins are sparse enough to gain: the fused
 // pass stages each sample once per (y, z) of its window and 16-coil group
-// (16 values and a few factors: about (8 + 3) * C bytes per (y, z)), where
-// the separate pass writes and reads the coil grids (16 * C * Ng bytes);
+// (16 values and a 32-byte record: about (8 + 2) * C bytes per (y, z)),
+// where the separate pass writes and reads the coil grids (16 * C * Ng
+// bytes);
 // the fused pass is used when the densest bin stages at most a quarter of
 // that (respiratory bins of a few percent of the lines stage a twentieth to
 // a fifth; a single frame with all lines about three quarters, which
@@ -536,30 +537,33 @@ int build_xlists(cgs_ctx* h) {
         total += n;
         mmax = std::max(mmax, n);
     }
-    const double staged = (double)mmax * 11.0 * h->C, round_trip = 16.0 * h->C * (double)h->ng;
+    const double staged = (double)mmax * 10.0 * h->C, round_trip = 16.0 * h->C * (double)h->ng;
     if (staged > 0.25 * round_trip) return CGS_OK;
+    const int rs = xrec_stride(kw);
     h->xl_n = total;
-    CKS(dalloc(h, &h->d_xidx, std::max(1LL, total)));
+    CKS(dalloc(h, &h->d_xrec, std::max(1LL, total * rs)));
     CKS(dalloc(h, &h->d_xoff, h->nbins * (ncell + 1)));
     unsigned *ktmp = nullptr, *kout = nullptr;
-    int* itmp = nullptr;
+    int *itmp = nullptr, *iout = nullptr;
     char* stmp = nullptr;
     const size_t sb = std::max<size_t>(1, xlist_sort_bytes(mmax));
     CKS(dalloc(h, &ktmp, std::max(1LL, mmax)));
     CKS(dalloc(h, &kout, std::max(1LL, mmax)));
     CKS(dalloc(h, &itmp, std::max(1LL, mmax)));
+    CKS(dalloc(h, &iout, std::max(1LL, mmax)));
     CKS(dalloc(h, &stmp, sb));
     long long g0 = 0;
     for (int b = 0; b < h->nbins; ++b) {
         const long long m = h->bin_off[b + 1] - h->bin_off[b];
         CK(build_xlists_bin(kw, h->p_base, h->p_vals, h->bin_off[b], m, h->N[1], h->N[2], ktmp, itmp, stmp, sb, kout,
-                            h->d_xidx + g0, h->d_xoff + b * (ncell + 1), g0, h->st));
+                            iout, h->d_xrec + g0 * rs, h->d_xoff + b * (ncell + 1), g0, h->st));
         g0 += m * kw * kw;
     }
     CK(hipStreamSynchronize(h->st));
     dfree(h, ktmp, std::max(1LL, mmax));
     dfree(h, kout, std::max(1LL, mmax));
     dfree(h, itmp, std::max(1LL, mmax));
+    dfree(h, iout, std::max(1LL, mmax));
     dfree(h, stmp, sb);
     h->xl_on = true;
     return CGS_OK;
@@ -789,9 +793,7 @@ int run_inv_fft(cgs_ctx* h, int b0, int nb, float2* q, const float2* p, double* 
         xa.x = x;
         xa.Y = xg_y;
         xa.gdcf = xg_dcf;
-        xa.vals = h->p_vals;
-        xa.base = h->p_base;
-        xa.idx = h->d_xidx;
+        xa.rec = h->d_xrec;
         xa.off = h->d_xoff;
         xa.C = h->C; xa.ny = h->N[1]; xa.nz = h->N[2];
         xa.thrf = h->kb.thr_up;

@@ -538,6 +538,28 @@ __device__ __forceinline__ float2* line_at(float2* g, long long ls, int i, int S
     else return g + (long long)(j + i * S) * ls + c;
 }
 
+// Zeros in columns [col_lo, col_lo + ncol) of the L lines (stride ls) of a
+// column group whose chunks no sample reaches.  The group's NCOL columns are
+// written row by row with 16-byte stores where the rows allow (even stride
+// and first column, full group, 16-byte aligned base), else element by
+// element.
+template <int L, int NCOL>
+__device__ __forceinline__ void zero_group(float2* g, long long ls, long long col_lo, int ncol) {
+    float2* const b = g + col_lo;
+    if (NCOL % 2 == 0 && ncol == NCOL && ((ls | col_lo) & 1) == 0 && (reinterpret_cast<uintptr_t>(g) & 15) == 0) {
+        constexpr int N2 = NCOL / 2 > 0 ? NCOL / 2 : 1;
+        for (int e = threadIdx.x; e < L * N2; e += CGS_THREADS) {
+            const int l = e / N2, c2 = e - (e / N2) * N2;
+            *reinterpret_cast<float4*>(b + (long long)l * ls + 2 * c2) = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    } else {
+        for (int e = threadIdx.x; e < L * ncol; e += CGS_THREADS) {
+            const int l = e / ncol, c = e - (e / ncol) * ncol;
+            b[(long long)l * ls + c] = make_float2(0.0f, 0.0f);
+        }
+    }
+}
+
 template <int L, int W, int DIR, int CPB, int AM>
 __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
     __shared__ float2 lds[L * W];
@@ -563,14 +585,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
     const long long c_first = grp * CPB;
     const long long c_end = min(c_first + CPB, nchunk);
     if (a.gact && !a.gact[outer * ngrp + grp]) {   // no sample reaches these columns: zeros
-        for (long long chunk = c_first; chunk < c_end; ++chunk) {
-            float2* g = gout + chunk * W;
-            const int cm = (int)min((long long)W, ncols - chunk * W) - 1;
-            for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-                const int l = e / W, c = e % W;
-                if (c <= cm) g[(long long)l * ls + c] = make_float2(0.0f, 0.0f);
-            }
-        }
+        zero_group<L, W * CPB>(gout, ls, c_first * W, (int)(min(c_end * W, ncols) - c_first * W));
         return;
     }
     for (int e = threadIdx.x; e < L; e += CGS_THREADS) twl[e] = a.tw[e];
@@ -629,30 +644,40 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
 // when given), w = float((vx * vy) * vz), added when w >= thrf -- the
 // weights and the tap set of gridding (Reco.cpp:492-504; Reco.h:218-242).
 // The cell's listed samples are staged in LDS, in the memory of the FFT tile
-// before the tile is filled: per sample its bx, its KW weights along x for
-// this cell and its 16 coil values; each first-stage input of a thread then
-// finds its samples by a binary search on bx and sums them from LDS.  The
-// chunk runs through the same stages and stores as the plain inverse x pass.
+// before the tile is filled: per sample its bx and its KW weights along x
+// for this cell (read from the entry's record, computed when the lists were
+// built) and its 16 coil values; a table of the entry run reaching each line
+// x (found by binary searches on bx) follows, and each first-stage input of a
+// thread sums its run from LDS.  The chunk runs through the same stages and
+// stores as the plain inverse x pass.
 // Column groups that no sample reaches (a.gact) are written as zeros.
 template <int L, int W, int CPB, int AM, int KW>
 __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
     const PlainArgs& a = xa.x;
-    constexpr int VS = val_stride(KW);
+    constexpr int RS = xrec_stride(KW);
     constexpr int NS = plan_nstages(L);
     static_assert(NS > 1, "fused gridding needs a multi-stage line transform");
     static_assert(W == 16, "fused gridding stages 16 coil values per sample");
     using FR = FirstRegs<L, W, CGS_THREADS>;
     constexpr int R0 = FR::R;
     constexpr int SL = plan_ls(L, NS - 1);
-    // staged samples per batch: bx (int), KW weights (float), W values (float2)
-    constexpr int NEB = (L * W * 8) / (4 + 4 * KW + 8 * W);
+    // staged samples per batch: bx (int), KW weights (float), sample index
+    // (int), W values (float2); then, per line x, the range of staged entries
+    // that reach it
+    constexpr int NEB = (L * W * 8 - 4 * L) / (8 + 4 * KW + 8 * W);
     constexpr int NE = (NEB / 16) * 16 < CGS_THREADS ? (NEB / 16) * 16 : CGS_THREADS;
     static_assert(NE >= 16, "tile too small to stage samples");
+    // coil values in flight per thread while staging: 8 where the LDS of a
+    // block already limits a CU to 2 blocks (8 waves, registers to spare),
+    // else 2 (16 waves of at most 128 VGPRs)
+    constexpr int NU = (160 * 1024) / (L * W * 8 + L * 8) <= 2 ? 8 : 2;
     __shared__ float2 lds[L * W];
     __shared__ float2 twl[L];
     float2* const ey = lds;                                        // [NE][W]
     float* const ew = reinterpret_cast<float*>(lds + NE * W);      // [NE][KW]
     int* const ebx = reinterpret_cast<int*>(ew + NE * KW);         // [NE]
+    int* const esid = ebx + NE;                                    // [NE]
+    int* const xr = esid + NE;                                     // [L] first | end << 16
     const long long nchunk = (a.ncols + W - 1) / W;
     const long long ngrp = (nchunk + CPB - 1) / CPB;
     const long long grp = blockIdx.x % ngrp, outer = blockIdx.x / ngrp;   // outer: bin of the launch
@@ -662,25 +687,31 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
     const long long c_first = grp * CPB;
     const long long c_end = min(c_first + CPB, nchunk);
     if (a.gact && !a.gact[outer * ngrp + grp]) {   // no sample reaches these columns: zeros
-        for (long long chunk = c_first; chunk < c_end; ++chunk) {
-            float2* g = gout + chunk * W;
-            const int cm = (int)min((long long)W, a.ncols - chunk * W) - 1;
-            for (int e = threadIdx.x; e < L * W; e += CGS_THREADS) {
-                const int l = e / W, c = e % W;
-                if (c <= cm) g[(long long)l * ls + c] = make_float2(0.0f, 0.0f);
-            }
-        }
+        zero_group<L, W * CPB>(gout, ls, c_first * W, (int)(min(c_end * W, a.ncols) - c_first * W));
         return;
     }
     for (int e = threadIdx.x; e < L; e += CGS_THREADS) twl[e] = a.tw[e];
     const int ncell = xa.ny * xa.nz;
     const long long* __restrict__ offb = xa.off + (long long)(xa.b0 + outer) * (ncell + 1);
     const int tid = threadIdx.x;
+    // list offsets of the group's cells (W <= C: at most CPB cells), loaded
+    // before any store of the block (a later wait for them would also wait
+    // for the stores issued before it)
+    const int cell_lo = (int)((c_first * W) / xa.C);
+    long long offs[CPB + 1];
+#pragma unroll
+    for (int k = 0; k <= CPB; ++k) {   // uniform: kept in scalar registers
+        const long long v = cell_lo + k <= ncell ? offb[cell_lo + k] : 0;
+        offs[k] = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
+                              (unsigned)__builtin_amdgcn_readfirstlane((int)v));
+    }
     for (long long chunk = c_first; chunk < c_end; ++chunk) {
         const long long col0 = chunk * W;
         const int cell = (int)(col0 / xa.C), c0 = (int)(col0 % xa.C);
-        const int y = cell / xa.nz, z = cell - (cell / xa.nz) * xa.nz;
-        const long long e0 = offb[cell], e1 = offb[cell + 1];
+        long long e0 = offs[0], e1 = offs[1];
+#pragma unroll
+        for (int k = 1; k < CPB; ++k)
+            if (cell - cell_lo == k) { e0 = offs[k]; e1 = offs[k + 1]; }
         FR next;
 #pragma unroll
         for (int p = 0; p < FR::PER; ++p)
@@ -688,42 +719,79 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
             for (int i = 0; i < R0; ++i) next.v[p][i] = make_float2(0.0f, 0.0f);
         for (long long eb = e0; eb < e1; eb += NE) {
             const int ne = (int)min((long long)NE, e1 - eb);
-            // stage: weights along x of this cell (one thread per sample), then
-            // the coil values (W threads per sample, coalesced)
+            // stage: the entries' records (one thread per entry), then the
+            // coil values (W threads per entry, coalesced, NU per thread in
+            // flight; the first NU are loaded while the run table is built)
             if (tid < ne) {
-                const int sidx = xa.idx[eb + tid];
-                const int4 b = xa.base[sidx];
-                const double* v = xa.vals + (long long)sidx * VS;
-                const double vy = v[KW + (y - b.y)], vz = v[2 * KW + (z - b.z)];
-                ebx[tid] = b.x;
+                const int4* r = reinterpret_cast<const int4*>(xa.rec + (eb + tid) * RS);
+                int rr[RS];
 #pragma unroll
-                for (int ox = 0; ox < KW; ++ox) ew[tid * KW + ox] = (float)((v[ox] * vy) * vz);
-            }
-            for (int t = tid; t < ne * W; t += CGS_THREADS) {
-                const int e = t / W, c = t % W;
-                const int sidx = xa.idx[eb + e];
-                float2 yv = xa.Y[(long long)sidx * xa.C + c0 + c];
-                if (xa.gdcf) yv = cscale(yv, xa.gdcf[sidx]);
-                ey[t] = yv;
+                for (int q = 0; q < RS / 4; ++q) {
+                    const int4 v = r[q];
+                    rr[4 * q] = v.x; rr[4 * q + 1] = v.y; rr[4 * q + 2] = v.z; rr[4 * q + 3] = v.w;
+                }
+                ebx[tid] = rr[0];
+#pragma unroll
+                for (int ox = 0; ox < KW; ++ox) ew[tid * KW + ox] = __int_as_float(rr[1 + ox]);
+                esid[tid] = rr[KW + 1];
             }
             __syncthreads();
+            const int nv = ne * W;
+            float2 yv[NU];
+            auto load_values = [&](int t0) {
+#pragma unroll
+                for (int q = 0; q < NU; ++q) {
+                    const int t = t0 + q * CGS_THREADS + tid;
+                    if (t < nv) {
+                        const int sidx = esid[t / W];
+                        yv[q] = xa.Y[(long long)sidx * xa.C + c0 + t % W];
+                        if (xa.gdcf) yv[q] = cscale(yv[q], xa.gdcf[sidx]);
+                    }
+                }
+            };
+            auto store_values = [&](int t0) {
+#pragma unroll
+                for (int q = 0; q < NU; ++q) {
+                    const int t = t0 + q * CGS_THREADS + tid;
+                    if (t < nv) ey[t] = yv[q];
+                }
+            };
+            load_values(0);
+            // entries reaching line x (x - KW < bx <= x) are the sorted run
+            // [first bx >= x - KW + 1, first bx > x): two searches per line,
+            // shared by the line's W columns
+            for (int x = tid; x < L; x += CGS_THREADS) {
+                int lo = 0, hi = ne;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (ebx[mid] < x - KW + 1) lo = mid + 1; else hi = mid;
+                }
+                int en = lo;
+                hi = ne;
+                while (en < hi) {
+                    const int mid = (en + hi) >> 1;
+                    if (ebx[mid] <= x) en = mid + 1; else hi = mid;
+                }
+                xr[x] = lo | (en << 16);
+            }
+            store_values(0);
+            for (int t0 = NU * CGS_THREADS; t0 < nv; t0 += NU * CGS_THREADS) {
+                load_values(t0);
+                store_values(t0);
+            }
+            __syncthreads();
+            // each line adds its run in ascending entry order
 #pragma unroll
             for (int p = 0; p < FR::PER; ++p) {
                 const int b = tid + p * CGS_THREADS;
                 if ((FR::NB % CGS_THREADS) == 0 || b < FR::NB) {
                     const int col = b % W, j = b / W;
-                    int lo = 0;
 #pragma unroll
                     for (int i = 0; i < R0; ++i) {
                         const int x = j + i * (L / R0);
-                        // first staged sample with bx >= x - KW + 1 (lines ascend in i)
-                        int hi = ne;
-                        while (lo < hi) {
-                            const int mid = (lo + hi) >> 1;
-                            if (ebx[mid] < x - KW + 1) lo = mid + 1; else hi = mid;
-                        }
+                        const int r = xr[x];
                         float2 acc = next.v[p][i];
-                        for (int e = lo; e < ne && ebx[e] <= x; ++e) {
+                        for (int e = r & 0xffff; e < (r >> 16); ++e) {
                             const float w = ew[e * KW + (x - ebx[e])];
                             if (w >= xa.thrf) {
                                 const float2 yv = ey[e * W + col];

@@ -56,9 +56,7 @@ struct XGridArgs {
     PlainArgs x;
     const float2* Y;        // [Mtot][C] sample values
     const float* gdcf;      // multiply Y by these weights first (or null), as gridding does
-    const double* vals;     // [Mtot][VS] per-axis KB factors
-    const int4* base;       // [Mtot] window origins
-    const int* idx;         // sorted per-cell lists: global sample indices
+    const int* rec;         // sorted per-cell lists: [entries][xrec_stride(KW)] records
     const long long* off;   // [nbins_total][ny * nz + 1] list positions
     int C, ny, nz;
     float thrf;             // KbParams::thr_up
@@ -156,19 +154,24 @@ hipError_t launch_grid(int kw, int tcfg, const GridArgs& a, hipStream_t s);
 hipError_t launch_grid_lists(int kw, int tcfg, const GridArgs& a, int* counts, hipStream_t s);
 
 // Per-cell sample lists of the fused gridding (cgs_xlists.hip): keys
-// (y * nz + z) << kXListBits | (bx + kXListOff), sorted, with the samples'
-// global indices; off[bin][cell] (absolute positions) of ny * nz + 1 cells.
+// (y * nz + z) << kXListBits | (bx + kXListOff), sorted, and per sorted
+// entry a record of xrec_stride(kw) ints: bx, the kw weights along x of the
+// entry's cell (float bits, w[ox] = float((vx[ox] * vy) * vz)) and the
+// sample's global index; off[bin][cell] (absolute positions) of ny * nz + 1
+// cells.
 constexpr int kXListBits = 12;
 constexpr int kXListOff = 16;
+__host__ __device__ constexpr int xrec_stride(int kw) { return (kw + 2 + 3) / 4 * 4; }
 bool xlist_supported(int nx, int ny, int nz);
 // hipCUB radix-sort workspace for n keys
 size_t xlist_sort_bytes(long long n);
 // The lists of one bin: its m samples from global index first; keys_tmp /
-// idx_tmp of m * kw * kw entries; the sorted keys and indices go to
-// keys_out / idx_out (same size) and the cell offsets, plus off0, to off.
+// idx_tmp / keys_out / idx_out of m * kw * kw entries (work space); the
+// records go to rec (same number of records) and the cell offsets, plus
+// off0, to off.
 hipError_t build_xlists_bin(int kw, const int4* base, const double* vals, long long first, long long m, int ny, int nz,
                             unsigned* keys_tmp, int* idx_tmp, void* sort_tmp, size_t sort_bytes, unsigned* keys_out,
-                            int* idx_out, long long* off, long long off0, hipStream_t s);
+                            int* idx_out, int* rec, long long* off, long long off0, hipStream_t s);
 
 // Coordinate gradient of the sample values: for every sample i (sorted
 // position s, original index base[s].w)

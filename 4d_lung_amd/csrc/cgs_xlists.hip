@@ -11,8 +11,10 @@
 // trajectory is fixed across CG iterations, so the lists are built once when
 // the bins are packed: every sample emits a key (cell, bx) per (y, z) of its
 // window whose y and z factors are non-zero, a stable radix sort orders the
-// keys (ties keep the sample order), and each cell's first entry is found by
-// a binary search.
+// keys (ties keep the sample order), each sorted entry gets a record with
+// everything the fused pass needs but the sample's values (bx, its float
+// weights along x for that cell, the sample index), and each cell's first
+// entry is found by a binary search.
 #include <hipcub/hipcub.hpp>
 
 #include "cgs_internal.h"
@@ -41,6 +43,37 @@ __global__ void k_xlist_keys(const int4* __restrict__ base, const double* __rest
         key = ((unsigned)(y * nz + z) << kXListBits) | (unsigned)(b.x + kXListOff);
     keys[t] = key;
     idx[t] = (int)s;
+}
+
+// Record of sorted entry t (a tap of a sample in cell (y, z) = key >> XBITS):
+// bx, the KW weights along x times the cell's y and z factors, rounded to
+// float as the gridding kernels round them, and the sample index.  Entries
+// with the empty key sort after every cell and are never read.
+template <int KW>
+__global__ void k_xlist_records(const unsigned* __restrict__ keys, const int* __restrict__ idx, long long n,
+                                const int4* __restrict__ base, const double* __restrict__ vals, int nz,
+                                int* __restrict__ rec) {
+    constexpr int VS = val_stride(KW);
+    constexpr int RS = xrec_stride(KW);
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const unsigned key = keys[t];
+    if (key == 0xffffffffu) return;
+    const int cell = (int)(key >> kXListBits), y = cell / nz, z = cell - (cell / nz) * nz;
+    const int sidx = idx[t];
+    const int4 b = base[sidx];
+    const double* v = vals + (long long)sidx * VS;
+    const double vy = v[KW + (y - b.y)], vz = v[2 * KW + (z - b.z)];
+    int r[RS];
+    r[0] = b.x;
+#pragma unroll
+    for (int ox = 0; ox < KW; ++ox) r[1 + ox] = __float_as_int((float)((v[ox] * vy) * vz));
+    r[KW + 1] = sidx;
+#pragma unroll
+    for (int q = KW + 2; q < RS; ++q) r[q] = 0;
+    int4* o = reinterpret_cast<int4*>(rec + t * RS);
+#pragma unroll
+    for (int q = 0; q < RS / 4; ++q) o[q] = make_int4(r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]);
 }
 
 // off[c] = first sorted position with a key of cell c or above (c <= ncell).
@@ -74,7 +107,7 @@ size_t xlist_sort_bytes(long long n) {
 
 hipError_t build_xlists_bin(int kw, const int4* base, const double* vals, long long first, long long m, int ny, int nz,
                             unsigned* keys_tmp, int* idx_tmp, void* sort_tmp, size_t sort_bytes, unsigned* keys_out,
-                            int* idx_out, long long* off, long long off0, hipStream_t s) {
+                            int* idx_out, int* rec, long long* off, long long off0, hipStream_t s) {
     const int ncell = ny * nz;
     const long long n = m * kw * kw;
     hipError_t e = hipSuccess;
@@ -93,6 +126,12 @@ hipError_t build_xlists_bin(int kw, const int4* base, const double* vals, long l
         if ((e = hipcub::DeviceRadixSort::SortPairs(sort_tmp, tb, keys_tmp, keys_out, idx_tmp, idx_out, (int)n, 0, 32,
                                                     s)) != hipSuccess)
             return e;
+        switch (kw) {
+            case 3: k_xlist_records<3><<<blocks, 256, 0, s>>>(keys_out, idx_out, n, base, vals, nz, rec); break;
+            case 5: k_xlist_records<5><<<blocks, 256, 0, s>>>(keys_out, idx_out, n, base, vals, nz, rec); break;
+            case 7: k_xlist_records<7><<<blocks, 256, 0, s>>>(keys_out, idx_out, n, base, vals, nz, rec); break;
+        }
+        if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     k_xlist_offsets<<<(unsigned)((ncell + 1 + 255) / 256), 256, 0, s>>>(keys_out, n, ncell, off, off0);
     return hipGetLastError();

@@ -7,7 +7,8 @@ TRACE_DIR holds ``run_kernel_trace.csv`` of ``rocprofv3 --kernel-trace
 --stats -f csv -o run``; FETCH_DIR / WRITE_DIR hold ``run_counter_collection.csv``
 of ``rocprofv3 --pmc FETCH_SIZE`` / ``--pmc WRITE_SIZE`` runs of
 ``bench.py --calibrate-copy``.  Kernels are named by CG stage (the forward
-strided passes run y then x, the inverse ones x then y).  The counters are
+strided passes run y then x, the inverse ones x then y; a fused gridding
+launch, k_fft_xgrid, is the inverse x pass of its transform).  The counters are
 scaled by what they read for the calibration copy (1 GiB read and 1 GiB
 written by the runtime's copy kernel).  Per stage the median over the run's
 launches is reported (setup launches of the same kernels are outnumbered by
@@ -40,9 +41,16 @@ def stage_of(name, counters):
         return "fft_z_fwd"
     if "k_fft_zpost" in n:
         return "fft_z_inv"
+    if "k_fft_xgrid" in n:
+        # fused gridding + inverse x pass: the inverse transform's next plain
+        # launch is its y pass
+        counters["xgrid"] = True
+        return "fft_x_inv_grid"
     m = re.search(r"k_fft_plain<(\d+), (\d+), (-?1)", n)
     if m:
         d = int(m.group(3))
+        if d > 0 and counters.pop("xgrid", False):
+            return "fft_y_inv"
         k = counters[d]
         counters[d] += 1
         if d < 0:

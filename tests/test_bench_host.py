@@ -81,6 +81,17 @@ def test_stage_names_of_kernel_trace():
     got = [prof.stage_of(n, cnt) for n in names]
     assert got == ["fft_y_fwd", "fft_x_fwd", "fft_x_inv", "fft_y_inv", "fft_z_fwd", "fft_z_inv", "grid", "degrid",
                    None]
+    # fused gridding: the x pass of the inverse transform is k_fft_xgrid, so
+    # the one plain inverse launch after it is the y pass; a later unfused
+    # transform alternates x, y again
+    cnt = {-1: 0, 1: 0}
+    names = ["cgs::k_fft_xgrid<256, 16, 4, 2, 5>", "cgs::k_fft_plain<256, 32, 1, 6, 0>",
+             "cgs::k_fft_zpost_mb<192, 8, 256>", "cgs::k_fft_xgrid<256, 16, 4, 2, 5>",
+             "cgs::k_fft_plain<256, 32, 1, 6, 0>", "cgs::k_grid<5, 4, 8, 8, 16, 2>",
+             "cgs::k_fft_plain<256, 16, 1, 4, 0>", "cgs::k_fft_plain<256, 32, 1, 6, 0>"]
+    got = [prof.stage_of(n, cnt) for n in names]
+    assert got == ["fft_x_inv_grid", "fft_y_inv", "fft_z_inv", "fft_x_inv_grid", "fft_y_inv", "grid", "fft_x_inv",
+                   "fft_y_inv"]
 
 
 def test_kernel_summary_keeps_its_description(tmp_path):
