@@ -663,18 +663,27 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
     constexpr int SL = plan_ls(L, NS - 1);
     // staged samples per batch: bx (int), KW weights (float), sample index
     // (int), W values (float2); then, per line x, the range of staged entries
-    // that reach it
+    // that reach it.  All in the tile's memory, or -- where a side region of
+    // LDS costs no block per CU (kSide) -- everything but the values in a
+    // region of their own, which outlives the chunk's transform: the next
+    // chunk of the same cell (C > W) then stages only its values.
+    constexpr int kLdsCu = 160 * 1024, kBase = L * W * 8 + L * 8;
+    constexpr int NES = L < CGS_THREADS ? (L / 16) * 16 : CGS_THREADS;
+    constexpr int kSideBytes = NES * (8 + 4 * KW) + 4 * L;
+    constexpr bool kSide = kLdsCu / (kBase + kSideBytes + 64) == kLdsCu / kBase;
     constexpr int NEB = (L * W * 8 - 4 * L) / (8 + 4 * KW + 8 * W);
-    constexpr int NE = (NEB / 16) * 16 < CGS_THREADS ? (NEB / 16) * 16 : CGS_THREADS;
+    constexpr int NE = kSide ? NES : ((NEB / 16) * 16 < CGS_THREADS ? (NEB / 16) * 16 : CGS_THREADS);
     static_assert(NE >= 16, "tile too small to stage samples");
     // coil values in flight per thread while staging: 8 where the LDS of a
     // block already limits a CU to 2 blocks (8 waves, registers to spare),
     // else 2 (16 waves of at most 128 VGPRs)
-    constexpr int NU = (160 * 1024) / (L * W * 8 + L * 8) <= 2 ? 8 : 2;
+    constexpr int NU = kLdsCu / kBase <= 2 ? 8 : 2;
     __shared__ float2 lds[L * W];
     __shared__ float2 twl[L];
+    __shared__ int side[kSide ? kSideBytes / 4 : 1];
     float2* const ey = lds;                                        // [NE][W]
-    float* const ew = reinterpret_cast<float*>(lds + NE * W);      // [NE][KW]
+    float* const ew = kSide ? reinterpret_cast<float*>(side)       // [NE][KW]
+                            : reinterpret_cast<float*>(lds + NE * W);
     int* const ebx = reinterpret_cast<int*>(ew + NE * KW);         // [NE]
     int* const esid = ebx + NE;                                    // [NE]
     int* const xr = esid + NE;                                     // [L] first | end << 16
@@ -705,6 +714,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
         offs[k] = (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(v >> 32)) << 32) |
                               (unsigned)__builtin_amdgcn_readfirstlane((int)v));
     }
+    int kept = -1;   // cell whose (single) batch of records and run table the side region holds
     for (long long chunk = c_first; chunk < c_end; ++chunk) {
         const long long col0 = chunk * W;
         const int cell = (int)(col0 / xa.C), c0 = (int)(col0 % xa.C);
@@ -712,6 +722,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
 #pragma unroll
         for (int k = 1; k < CPB; ++k)
             if (cell - cell_lo == k) { e0 = offs[k]; e1 = offs[k + 1]; }
+        const bool reuse = kSide && cell == kept;
         FR next;
 #pragma unroll
         for (int p = 0; p < FR::PER; ++p)
@@ -722,7 +733,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
             // stage: the entries' records (one thread per entry), then the
             // coil values (W threads per entry, coalesced, NU per thread in
             // flight; the first NU are loaded while the run table is built)
-            if (tid < ne) {
+            if (!reuse && tid < ne) {
                 const int4* r = reinterpret_cast<const int4*>(xa.rec + (eb + tid) * RS);
                 int rr[RS];
 #pragma unroll
@@ -760,7 +771,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
             // entries reaching line x (x - KW < bx <= x) are the sorted run
             // [first bx >= x - KW + 1, first bx > x): two searches per line,
             // shared by the line's W columns
-            for (int x = tid; x < L; x += CGS_THREADS) {
+            for (int x = reuse ? L : tid; x < L; x += CGS_THREADS) {
                 int lo = 0, hi = ne;
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
@@ -805,6 +816,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
             }
             __syncthreads();   // staged samples read (the memory is the tile's)
         }
+        kept = kSide && e1 > e0 && e1 - e0 <= NE ? cell : -1;
         const int cmax = (int)min((long long)W, a.ncols - col0) - 1;
         next.template run<+1, W>(lds, [&](int l, int c, float2 v) {
             if (c > cmax) v = make_float2(0.0f, 0.0f);

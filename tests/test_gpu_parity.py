@@ -424,6 +424,70 @@ def test_fused_gridding_matches_separate_pass_and_oracle(pkg, orc, wide16):
     o.exclude_missing_data(P["data"])
 
 
+@pytest.mark.parametrize("w,b,kw,ncoil", [(2.5, 5.2, 3, 16), (6.0, 13.9, 7, 16), (4.25, 14.9086, 5, 32)])
+def test_fused_gridding_kernel_widths_and_coil_groups(pkg, orc, w, b, kw, ncoil):
+    """The fused gridding with a 3- and a 7-point window (its lists and
+    records per window width) and with 32 coils (two 16-coil chunks per cell
+    sharing one list): eight bins of an even line split, solved fused and
+    with the separate gridding pass (batched and plain), and bin 0 against
+    the FP64 oracle built with the same (w, b).  Bars as for the other kernel
+    widths, from the reference's own FP32 arithmetic on bin 0 (its distance
+    from FP64, 4e-5 for the 7-point window, whose deapodization magnifies
+    FP32 rounding): the GPU images at most 3x as far from FP64 (and never
+    beyond the default bar), the two gridding orders at most 3x that far
+    apart in every bin (two FP32 results each about that far from FP64 can
+    be up to twice as far from each other)."""
+    P = small_problem(orc, grid=(64, 128, 96), image=(32, 64, 48), samp=(64, 64, 48), ncoil=ncoil, R=(2, 2),
+                      cycles=3, amp=3.0, seed=91)
+    nb = 8
+    lines = P["lines"]
+    acq = np.flatnonzero(lines.ravel())
+    labels = np.full(lines.size, -1)
+    labels[acq] = np.arange(acq.size) % nb
+    labels = labels.reshape(lines.shape)
+    bins = []
+    for k in range(nb):
+        sm = np.broadcast_to((labels == k)[None], (P["samp"][0],) + lines.shape).reshape(-1)
+        bins.append(np.where(sm[None, :], P["data"], 0).astype(np.complex64))
+    res = {}
+    for fused in (1, 0):
+        ctx = pkg.Context(P["grid"], P["samp"], P["ncoil"])
+        ctx.set_kernel(w, b)
+        ctx.set_trajectory(P["traj"])
+        ctx.calculate_weights()
+        ctx.exclude_missing_data(P["data"])
+        ctx.calculate_kernel_ft()
+        ctx.set_maps(P["maps"])
+        assert ctx.kernel_info()[0] == kw
+        ctx.set_fused_gridding(fused)
+        ctx.set_num_bins(nb)
+        for k, d in enumerate(bins):
+            ctx.set_bin_data(k, d)
+        xb, _, ib = ctx.cg_sense_bins(mode=1, thresh=0.0, max_iter=7)
+        assert ctx.fused_gridding() == bool(fused)
+        xp, _, _ = ctx.cg_sense_bins(mode=0, thresh=0.0, max_iter=7)
+        assert all(int(i) == 8 for i in ib)
+        if fused:
+            np.testing.assert_array_equal(xb, xp)
+        res[fused] = xb
+        ctx.close()
+    o = orc.OracleReco(P["grid"], P["samp"], P["ncoil"], w=w, b=b)
+    o.set_trajectory(P["traj"])
+    o.calculate_weights(bins[0])
+    o.calculate_kernel_ft()
+    o.set_maps(P["maps"])
+    xr, _, _ = o.cg_sense(bins[0], thresh=0.0, max_iter=7, prec=64)
+    x32, _, _ = o.cg_sense(bins[0], thresh=0.0, max_iter=7, prec=32)
+    ref = rel_l2(x32, xr)
+    sep = max(rel_l2(res[1][k], res[0][k]) for k in range(nb))
+    print("%d-point window, %d coils: fused vs separate %.2e (max over bins), bin 0 fused / separate vs FP64 "
+          "%.2e / %.2e, reference FP32 vs FP64 %.2e" % (kw, ncoil, sep, rel_l2(res[1][0], xr), rel_l2(res[0][0], xr),
+                                                       ref), flush=True)
+    assert sep < max(1e-5, 3.0 * ref)
+    for fused in (1, 0):
+        assert rel_l2(res[fused][0], xr) < max(CG_TOL, 3.0 * ref)
+
+
 @pytest.mark.parametrize("coils", [5, 12, 16])
 def test_inverse_zpass_bin_groups_bit_identical(pkg, orc, coils):
     """The inverse z pass taking one cell line for several bins of a launch
