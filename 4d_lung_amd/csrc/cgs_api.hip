@@ -538,33 +538,48 @@ int build_xlists(cgs_ctx* h) {
         mmax = std::max(mmax, n);
     }
     const double staged = (double)mmax * 10.0 * h->C, round_trip = 16.0 * h->C * (double)h->ng;
-    if (staged > 0.25 * round_trip) return CGS_OK;
+    if (staged > 0.25 * round_trip || mmax >= (1LL << 31) - 256) return CGS_OK;   // (the sort takes int counts)
     const int rs = xrec_stride(kw);
-    h->xl_n = total;
-    CKS(dalloc(h, &h->d_xrec, std::max(1LL, total * rs)));
-    CKS(dalloc(h, &h->d_xoff, h->nbins * (ncell + 1)));
     unsigned *ktmp = nullptr, *kout = nullptr;
     int *itmp = nullptr, *iout = nullptr;
     char* stmp = nullptr;
     const size_t sb = std::max<size_t>(1, xlist_sort_bytes(mmax));
-    CKS(dalloc(h, &ktmp, std::max(1LL, mmax)));
-    CKS(dalloc(h, &kout, std::max(1LL, mmax)));
-    CKS(dalloc(h, &itmp, std::max(1LL, mmax)));
-    CKS(dalloc(h, &iout, std::max(1LL, mmax)));
-    CKS(dalloc(h, &stmp, sb));
+    auto free_work = [&]() {
+        dfree(h, ktmp, std::max(1LL, mmax));
+        dfree(h, kout, std::max(1LL, mmax));
+        dfree(h, itmp, std::max(1LL, mmax));
+        dfree(h, iout, std::max(1LL, mmax));
+        dfree(h, stmp, sb);
+    };
+    // the lists only speed up E^H: where their memory (about 32 bytes per
+    // sample tap) is not available, the separate gridding pass is used
+    h->xl_n = total;
+    if (dalloc(h, &h->d_xrec, std::max(1LL, total * rs)) != CGS_OK ||
+        dalloc(h, &h->d_xoff, h->nbins * (ncell + 1)) != CGS_OK || dalloc(h, &ktmp, std::max(1LL, mmax)) != CGS_OK ||
+        dalloc(h, &kout, std::max(1LL, mmax)) != CGS_OK || dalloc(h, &itmp, std::max(1LL, mmax)) != CGS_OK ||
+        dalloc(h, &iout, std::max(1LL, mmax)) != CGS_OK || dalloc(h, &stmp, sb) != CGS_OK) {
+        free_work();
+        dfree(h, h->d_xrec, std::max(1LL, total * rs));
+        dfree(h, h->d_xoff, h->nbins * (ncell + 1));
+        h->xl_n = 0;
+        h->err.clear();
+        return CGS_OK;
+    }
     long long g0 = 0;
     for (int b = 0; b < h->nbins; ++b) {
         const long long m = h->bin_off[b + 1] - h->bin_off[b];
-        CK(build_xlists_bin(kw, h->p_base, h->p_vals, h->bin_off[b], m, h->N[1], h->N[2], ktmp, itmp, stmp, sb, kout,
-                            iout, h->d_xrec + g0 * rs, h->d_xoff + b * (ncell + 1), g0, h->st));
+        hipError_t e = build_xlists_bin(kw, h->p_base, h->p_vals, h->bin_off[b], m, h->N[1], h->N[2], ktmp, itmp, stmp,
+                                        sb, kout, iout, h->d_xrec + g0 * rs, h->d_xoff + b * (ncell + 1), g0, h->st);
+        if (e == hipSuccess && b + 1 == h->nbins) e = hipStreamSynchronize(h->st);
+        if (e != hipSuccess) {
+            (void)hipStreamSynchronize(h->st);
+            free_work();
+            return fail_hip(h, "build_xlists_bin", e);
+        }
         g0 += m * kw * kw;
     }
-    CK(hipStreamSynchronize(h->st));
-    dfree(h, ktmp, std::max(1LL, mmax));
-    dfree(h, kout, std::max(1LL, mmax));
-    dfree(h, itmp, std::max(1LL, mmax));
-    dfree(h, iout, std::max(1LL, mmax));
-    dfree(h, stmp, sb);
+    if (h->nbins == 0) CK(hipStreamSynchronize(h->st));
+    free_work();
     h->xl_on = true;
     return CGS_OK;
 }
