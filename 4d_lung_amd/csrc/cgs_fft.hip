@@ -662,22 +662,19 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
     constexpr int R0 = FR::R;
     constexpr int SL = plan_ls(L, NS - 1);
     // staged samples per batch: bx (int), KW weights (float), sample index
-    // (int), W values (float2); then, per line x, the range of staged entries
-    // that reach it.  All in the tile's memory, or -- where a side region of
-    // LDS costs no block per CU (kSide) -- everything but the values in a
-    // region of their own, which outlives the chunk's transform: the next
-    // chunk of the same cell (C > W) then stages only its values.
+    // (int), density weight (float), W values (float2); then, per line x, the
+    // range of staged entries that reach it.  All in the tile's memory, or --
+    // where a side region of LDS costs no block per CU (kSide) -- everything
+    // but the values in a region of their own, which outlives the chunk's
+    // transform: the next chunk of the same cell (C > W) then stages only its
+    // values.
     constexpr int kLdsCu = 160 * 1024, kBase = L * W * 8 + L * 8;
     constexpr int NES = L < CGS_THREADS ? (L / 16) * 16 : CGS_THREADS;
-    constexpr int kSideBytes = NES * (8 + 4 * KW) + 4 * L;
+    constexpr int kSideBytes = NES * (12 + 4 * KW) + 4 * L;
     constexpr bool kSide = kLdsCu / (kBase + kSideBytes + 64) == kLdsCu / kBase;
-    constexpr int NEB = (L * W * 8 - 4 * L) / (8 + 4 * KW + 8 * W);
+    constexpr int NEB = (L * W * 8 - 4 * L) / (12 + 4 * KW + 8 * W);
     constexpr int NE = kSide ? NES : ((NEB / 16) * 16 < CGS_THREADS ? (NEB / 16) * 16 : CGS_THREADS);
     static_assert(NE >= 16, "tile too small to stage samples");
-    // coil values in flight per thread while staging: 8 where the LDS of a
-    // block already limits a CU to 2 blocks (8 waves, registers to spare),
-    // else 2 (16 waves of at most 128 VGPRs)
-    constexpr int NU = kLdsCu / kBase <= 2 ? 8 : 2;
     __shared__ float2 lds[L * W];
     __shared__ float2 twl[L];
     __shared__ int side[kSide ? kSideBytes / 4 : 1];
@@ -686,7 +683,8 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
                             : reinterpret_cast<float*>(lds + NE * W);
     int* const ebx = reinterpret_cast<int*>(ew + NE * KW);         // [NE]
     int* const esid = ebx + NE;                                    // [NE]
-    int* const xr = esid + NE;                                     // [L] first | end << 16
+    float* const edcf = reinterpret_cast<float*>(esid + NE);       // [NE]
+    int* const xr = reinterpret_cast<int*>(edcf + NE);             // [L] first | end << 16
     const long long nchunk = (a.ncols + W - 1) / W;
     const long long ngrp = (nchunk + CPB - 1) / CPB;
     const long long grp = blockIdx.x % ngrp, outer = blockIdx.x / ngrp;   // outer: bin of the launch
@@ -731,8 +729,9 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
         for (long long eb = e0; eb < e1; eb += NE) {
             const int ne = (int)min((long long)NE, e1 - eb);
             // stage: the entries' records (one thread per entry), then the
-            // coil values (W threads per entry, coalesced, NU per thread in
-            // flight; the first NU are loaded while the run table is built)
+            // coil values, straight from memory into LDS (16 bytes per lane, a
+            // wave writes 8 entries' 16 values contiguously: no registers,
+            // every load in flight while the run table is built)
             if (!reuse && tid < ne) {
                 const int4* r = reinterpret_cast<const int4*>(xa.rec + (eb + tid) * RS);
                 int rr[RS];
@@ -745,29 +744,21 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
 #pragma unroll
                 for (int ox = 0; ox < KW; ++ox) ew[tid * KW + ox] = __int_as_float(rr[1 + ox]);
                 esid[tid] = rr[KW + 1];
+                if (xa.gdcf) edcf[tid] = xa.gdcf[rr[KW + 1]];
             }
             __syncthreads();
-            const int nv = ne * W;
-            float2 yv[NU];
-            auto load_values = [&](int t0) {
-#pragma unroll
-                for (int q = 0; q < NU; ++q) {
-                    const int t = t0 + q * CGS_THREADS + tid;
-                    if (t < nv) {
-                        const int sidx = esid[t / W];
-                        yv[q] = xa.Y[(long long)sidx * xa.C + c0 + t % W];
-                        if (xa.gdcf) yv[q] = cscale(yv[q], xa.gdcf[sidx]);
+            {
+                static_assert(W == 16, "a wave stages 8 entries of 16 values");
+                const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+                for (int k = wave; k < (ne + 7) / 8; k += CGS_THREADS / 64) {
+                    const int e = k * 8 + (lane >> 3);
+                    if (e < ne) {
+                        const float2* src = xa.Y + (long long)esid[e] * xa.C + c0 + 2 * (lane & 7);
+                        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)(ey + k * 128),
+                                                         16, 0, 0);
                     }
                 }
-            };
-            auto store_values = [&](int t0) {
-#pragma unroll
-                for (int q = 0; q < NU; ++q) {
-                    const int t = t0 + q * CGS_THREADS + tid;
-                    if (t < nv) ey[t] = yv[q];
-                }
-            };
-            load_values(0);
+            }
             // entries reaching line x (x - KW < bx <= x) are the sorted run
             // [first bx >= x - KW + 1, first bx > x): two searches per line,
             // shared by the line's W columns
@@ -785,11 +776,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
                 }
                 xr[x] = lo | (en << 16);
             }
-            store_values(0);
-            for (int t0 = NU * CGS_THREADS; t0 < nv; t0 += NU * CGS_THREADS) {
-                load_values(t0);
-                store_values(t0);
-            }
+            __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0): the direct loads have reached LDS
             __syncthreads();
             // each line adds its run in ascending entry order
 #pragma unroll
@@ -805,7 +792,8 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
                         for (int e = r & 0xffff; e < (r >> 16); ++e) {
                             const float w = ew[e * KW + (x - ebx[e])];
                             if (w >= xa.thrf) {
-                                const float2 yv = ey[e * W + col];
+                                float2 yv = ey[e * W + col];
+                                if (xa.gdcf) yv = cscale(yv, edcf[e]);   // as gridding weights the values
                                 acc.x = fmaf(w, yv.x, acc.x);
                                 acc.y = fmaf(w, yv.y, acc.y);
                             }
