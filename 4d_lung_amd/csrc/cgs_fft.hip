@@ -661,28 +661,36 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
     using FR = FirstRegs<L, W, CGS_THREADS>;
     constexpr int R0 = FR::R;
     constexpr int SL = plan_ls(L, NS - 1);
-    // staged samples per batch: bx (int), KW weights (float), sample index
-    // (int), density weight (float), W values (float2); then, per line x, the
-    // range of staged entries that reach it.  All in the tile's memory, or --
-    // where a side region of LDS costs no block per CU (kSide) -- everything
-    // but the values in a region of their own, which outlives the chunk's
-    // transform: the next chunk of the same cell (C > W) then stages only its
-    // values.
+    // staged samples per batch: W values (float2) and bx (int), needed for
+    // the current chunk only; per entry its KW weights in a row of EWS slots
+    // (weight ox at slot (bx + ox) mod EWS: line x of the entry's run finds
+    // its weight at slot x mod EWS without reading bx), sample index (int),
+    // density weight (float); per line x the range of staged entries that
+    // reach it.  All in the tile's memory, or -- where a side region of LDS
+    // costs no block per CU (kSide) -- the per-entry rows and the line ranges
+    // in a region of their own, which outlives the chunk's transform: the
+    // next chunk of the same cell (C > W) then stages only its values.
+    constexpr int EWS = KW <= 4 ? 4 : 8;
+    constexpr int kMeta = 4 * EWS + 8;
     constexpr int kLdsCu = 160 * 1024, kBase = L * W * 8 + L * 8;
-    constexpr int NES = L < CGS_THREADS ? (L / 16) * 16 : CGS_THREADS;
-    constexpr int kSideBytes = NES * (12 + 4 * KW) + 4 * L;
-    constexpr bool kSide = kLdsCu / (kBase + kSideBytes + 64) == kLdsCu / kBase;
-    constexpr int NEB = (L * W * 8 - 4 * L) / (12 + 4 * KW + 8 * W);
-    constexpr int NE = kSide ? NES : ((NEB / 16) * 16 < CGS_THREADS ? (NEB / 16) * 16 : CGS_THREADS);
+    constexpr int kBudget = kLdsCu / (kLdsCu / kBase);   // per block, keeping the blocks per CU
+    constexpr int kCap = L < CGS_THREADS ? (L / 16) * 16 : CGS_THREADS;
+    constexpr int NES0 = ((kBudget - kBase - 4 * L - 64) / kMeta / 16) * 16;
+    constexpr int NES = NES0 < kCap ? NES0 : kCap;
+    constexpr bool kSide = NES >= 128;
+    constexpr int kSideBytes = kSide ? NES * kMeta + 4 * L : 4;
+    constexpr int NEB = (L * W * 8 - 4 * L) / (8 * W + 4 + kMeta);
+    constexpr int NE = kSide ? NES : ((NEB / 16) * 16 < kCap ? (NEB / 16) * 16 : kCap);
     static_assert(NE >= 16, "tile too small to stage samples");
+    static_assert(NE * (8 * W + 4) + (kSide ? 0 : NE * kMeta + 4 * L) <= L * W * 8, "staging exceeds the tile");
     __shared__ float2 lds[L * W];
     __shared__ float2 twl[L];
-    __shared__ int side[kSide ? kSideBytes / 4 : 1];
+    __shared__ int side[kSideBytes / 4];
     float2* const ey = lds;                                        // [NE][W]
-    float* const ew = kSide ? reinterpret_cast<float*>(side)       // [NE][KW]
-                            : reinterpret_cast<float*>(lds + NE * W);
-    int* const ebx = reinterpret_cast<int*>(ew + NE * KW);         // [NE]
-    int* const esid = ebx + NE;                                    // [NE]
+    int* const ebx = reinterpret_cast<int*>(lds + NE * W);         // [NE]
+    float* const ew = kSide ? reinterpret_cast<float*>(side)       // [NE][EWS]
+                            : reinterpret_cast<float*>(ebx + NE);
+    int* const esid = reinterpret_cast<int*>(ew + NE * EWS);       // [NE]
     float* const edcf = reinterpret_cast<float*>(esid + NE);       // [NE]
     int* const xr = reinterpret_cast<int*>(edcf + NE);             // [L] first | end << 16
     const long long nchunk = (a.ncols + W - 1) / W;
@@ -742,7 +750,7 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
                 }
                 ebx[tid] = rr[0];
 #pragma unroll
-                for (int ox = 0; ox < KW; ++ox) ew[tid * KW + ox] = __int_as_float(rr[1 + ox]);
+                for (int ox = 0; ox < KW; ++ox) ew[tid * EWS + ((rr[0] + ox) & (EWS - 1))] = __int_as_float(rr[1 + ox]);
                 esid[tid] = rr[KW + 1];
                 if (xa.gdcf) edcf[tid] = xa.gdcf[rr[KW + 1]];
             }
@@ -790,9 +798,9 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
                         const int r = xr[x];
                         float2 acc = next.v[p][i];
                         for (int e = r & 0xffff; e < (r >> 16); ++e) {
-                            const float w = ew[e * KW + (x - ebx[e])];
+                            const float w = ew[e * EWS + (x & (EWS - 1))];   // weight x - bx of entry e
+                            float2 yv = ey[e * W + col];
                             if (w >= xa.thrf) {
-                                float2 yv = ey[e * W + col];
                                 if (xa.gdcf) yv = cscale(yv, edcf[e]);   // as gridding weights the values
                                 acc.x = fmaf(w, yv.x, acc.x);
                                 acc.y = fmaf(w, yv.y, acc.y);
