@@ -636,6 +636,13 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_plain(PlainArgs a) {
     }
 }
 
+// Column groups of the fused gridding pass that one XCD takes in a row
+// (neighbouring cells share samples; 32 measured best, 0 keeps the plain
+// workgroup order).
+#ifndef CGS_XGRID_XG
+#define CGS_XGRID_XG 32
+#endif
+
 // Inverse x pass with the gridding of E^H fused in (columns = coil groups of
 // W = 16 coils of one (y, z) cell, C a multiple of 16).  Instead of reading a
 // gridded chunk, the block forms it: the value of line x, coil c of cell
@@ -695,7 +702,19 @@ __global__ __launch_bounds__(CGS_THREADS) void k_fft_xgrid(XGridArgs xa) {
     int* const xr = reinterpret_cast<int*>(edcf + NE);             // [L] first | end << 16
     const long long nchunk = (a.ncols + W - 1) / W;
     const long long ngrp = (nchunk + CPB - 1) / CPB;
+#if CGS_XGRID_XG > 0
+    // consecutive workgroups go round-robin to the 8 XCDs: runs of XG
+    // consecutive column groups (neighbouring cells, which read the same
+    // samples) are given to one XCD, so its L2 serves their shared samples
+    unsigned lin = blockIdx.x;
+    if (gridDim.x % (8u * CGS_XGRID_XG) == 0) {
+        const unsigned s8 = blockIdx.x >> 3;
+        lin = (s8 / CGS_XGRID_XG) * (8u * CGS_XGRID_XG) + (blockIdx.x & 7u) * CGS_XGRID_XG + s8 % CGS_XGRID_XG;
+    }
+    const long long grp = lin % ngrp, outer = lin / ngrp;   // outer: bin of the launch
+#else
     const long long grp = blockIdx.x % ngrp, outer = blockIdx.x / ngrp;   // outer: bin of the launch
+#endif
     if (a.done && a.done[outer]) return;
     float2* gout = a.g + outer * a.os;
     const long long ls = a.ls;
